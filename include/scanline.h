@@ -1,0 +1,190 @@
+/*
+ * scanline.h -- C ABI of the MI355X (gfx950) comb-scanline path renderer.
+ *
+ * This is the drop-in boundary for the reference's rasterizer interface
+ * RasterizerBase::VGRasterizer (gpu-scanline/src/rasterizer/shared/ras_base.h:133-378)
+ * as constructed by createRasterizer(PM_Cut_Mask_Comb_Scanline)
+ * (gpu-scanline/src/rasterizer/shared/ras_factory.h:18-32).  Every entry point below
+ * names the reference member it replaces.  Signatures use plain C types only: no
+ * exceptions cross this boundary, every call returns an SL_* status code, and the
+ * text of the last failure is available from sl_last_error().
+ *
+ * Threading: a handle is single-threaded (like the reference's class, which keeps
+ * static state in rasterizeImpl, ras_cut_mask_comb_scanline.cu:882-884); separate
+ * handles are independent.  All GPU work of a handle runs on the HIP stream passed to
+ * sl_rasterize (NULL = the default stream).
+ */
+#ifndef SCANLINE_MI355X_H
+#define SCANLINE_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (replace the reference's std::runtime_error throws:
+ *      ras_factory.h:28, cuda_cached_allocator.h:80-81, ras_qm_mask.cu:173) ---- */
+#define SL_OK             0
+#define SL_E_ARG          1  /* invalid argument or malformed scene                  */
+#define SL_E_OOM          2  /* device or host allocation failed                      */
+#define SL_E_HIP          3  /* a HIP runtime call or kernel launch failed            */
+#define SL_E_UNSUPPORTED  4  /* valid request the renderer does not implement         */
+#define SL_E_STATE        5  /* call made in the wrong state (e.g. render w/o scene)  */
+#define SL_E_NODEVICE     6  /* no usable GPU (the renderer has no CPU fallback)       */
+
+/* ---- curve type codes: VGCurveType (bezier_curve_type.h:6-18); vertex count = code & 7 */
+#define SL_CURVE_LINEAR    0x02
+#define SL_CURVE_QUADRATIC 0x03
+#define SL_CURVE_CUBIC     0x04
+#define SL_CURVE_RATIONAL  0x13
+
+/* ---- fill rules: VGFillRule (vg_container.h:43-46) ---- */
+#define SL_FILL_NONZERO  0
+#define SL_FILL_EVEN_ODD 1
+
+typedef struct sl_handle sl_handle;
+
+/*
+ * Scene in the reference's GPU structure-of-arrays form (VGInputCurveDataPack /
+ * VGInputPathDataPack, ras_base.h:34-112, filled by RasterizerBase::addVg,
+ * ras_base.cpp:70-382).  All pointers are host pointers.  Vertices are stored per
+ * curve (a curve of type T owns (T & 7) consecutive vertices starting at
+ * curve_vertex_pos), exactly as VGContainer::addLinear/.../addRational lay them out
+ * (vg_container.cpp:97-125).  Curves must be grouped by path with non-decreasing
+ * curve_path_id; every contour must be closed (the reference's loaders close open
+ * contours: rvg.cpp:383-396, svg_load.cpp:784-795).
+ */
+typedef struct sl_scene {
+    uint32_t        n_paths;
+    uint32_t        n_curves;
+    uint32_t        n_vertices;
+    const float    *vertex;            /* 2*n_vertices floats: x0,y0,x1,y1,...        */
+    const uint32_t *curve_vertex_pos;  /* [n_curves] first vertex of each curve       */
+    const uint8_t  *curve_type;        /* [n_curves] SL_CURVE_*                        */
+    const uint32_t *curve_path_id;     /* [n_curves] owning path, non-decreasing      */
+    const float    *curve_arc_w;       /* [n_curves] rational weight w1; may be NULL  */
+    const uint8_t  *fill_rule;         /* [n_paths]  SL_FILL_*                         */
+    const uint32_t *fill_color;        /* [n_paths]  RGBA8, r | g<<8 | b<<16 | a<<24   */
+} sl_scene;
+
+/* ElementNumber (ras_base.h:124-131), widened to 64 bits. */
+typedef struct sl_counts {
+    int64_t path;
+    int64_t curve;
+    int64_t vertex;
+    int64_t curve_fragment;   /* raw fragments F (one per intersection entry)       */
+    int64_t merged_fragment;  /* merged (path, 2x2 cell) fragments M                 */
+    int64_t span;             /* interior spans S                                    */
+} sl_counts;
+
+/* ---- lifecycle -------------------------------------------------------------------- */
+
+/* createRasterizer(PM_Cut_Mask_Comb_Scanline) + VGRasterizer::init()
+ * (ras_factory.h:18-32; ras_cut_mask_comb_scanline.cpp:31-45).  Selects device 0
+ * unless `device` >= 0.  Fails with SL_E_NODEVICE when no GPU is present. */
+int sl_create(int device, sl_handle **out);
+
+/* VGRasterizer::uninit() + delete. */
+int sl_destroy(sl_handle *h);
+
+/* ---- configuration (ras_base.h:155-205) ------------------------------------------ */
+
+/* setSamples (ras_base.h:199). 8 and 32 follow the reference; 16 uses the layout
+ * documented in DESIGN.md (one 32-bit word per pixel column). */
+int sl_set_samples(sl_handle *h, int samples);
+
+/* setMultisampleOutput (ras_base.h:200): 0 = coverage-alpha compositing
+ * (output_{8,32}.frag.glsl), 1 = per-sample compositing + resolve
+ * (ms_output_{8,32}.frag.glsl, ras_base.cpp:1148-1163). */
+int sl_set_multisample_output(sl_handle *h, int enable);
+
+/* enableSRGBCorrection (ras_base.h:184): sRGB-decoded paint blended in linear space
+ * into an sRGB8 framebuffer (output_32.vert.glsl:44-51, .cu:1384-1389). */
+int sl_enable_srgb_correction(sl_handle *h, int enable);
+
+/* setInputTransform (ras_base.cpp:576-583). `m` is a column-major 4x4 matrix
+ * (glm layout); row i of the transform is (m[i], m[4+i], m[8+i], m[12+i]). */
+int sl_set_input_transform(sl_handle *h, const float m[16]);
+
+/* resizeOutputBuffer (ras_base.cpp:393-402). */
+int sl_resize_output_buffer(sl_handle *h, int width, int height);
+
+/* ---- scene ------------------------------------------------------------------------ */
+
+/* addVg (ras_cut_mask_comb_scanline.cpp:224-229 -> ras_base.cpp:70-382) followed by
+ * unifyCurveVertexOrder (ras_base.cpp:523-569).  The scene is validated, copied to the
+ * device and oriented; the caller's arrays are not retained. */
+int sl_add_vg(sl_handle *h, const sl_scene *scene);
+
+/* clear (ras_base.cpp:572-573): drop the uploaded scene. */
+int sl_clear(sl_handle *h);
+
+/* ---- rendering ---------------------------------------------------------------------- */
+
+/* rasterize() (ras_base.cpp:1131-1195 -> rasterizeImpl<2>, .cu:879-1453): renders the
+ * scene into the handle's device RGBA8 image (row 0 = y in [0,1), the reference's
+ * GL framebuffer order).  `stream` is a hipStream_t or NULL. */
+int sl_rasterize(sl_handle *h, void *stream);
+
+/* Device pointer and row pitch (bytes) of the handle's RGBA8 output image. */
+int sl_output_device_ptr(sl_handle *h, void **dev_ptr, int *pitch_bytes);
+
+/* Copy the last image to host memory (width*height*4 bytes).  flags bit 0: flip rows
+ * and force alpha to 255 as saveOutputToFile does (ras_base.cpp:1249-1262). */
+int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags);
+
+/* elementNumber() (ras_base.h:207). */
+int sl_element_number(sl_handle *h, sl_counts *out);
+
+/* enableStepTiming / stepTiming (ras_base.h:193-195): per-stage device times in
+ * milliseconds of the last sl_rasterize, measured with HIP events. */
+int sl_enable_step_timing(sl_handle *h, int enable);
+int sl_step_timing(sl_handle *h, double *ms, int capacity, int *n_stages);
+const char *sl_step_name(int stage);
+
+/* countPixel / pixelCount (ras_base.h:181-182; .cu:1269-1288): sum over draw items of
+ * 2 * item width. */
+int sl_pixel_count(sl_handle *h, int64_t *out);
+
+/* Text of the last error on this handle (never NULL). */
+const char *sl_last_error(sl_handle *h);
+
+/* ---- parity dumps (debug data of the reference's pipeline stages) --------------------
+ * After sl_debug_enable(h, 1), sl_rasterize also materialises the intermediate arrays
+ * in the reference's own layouts, and sl_debug_dump copies one of them to host memory:
+ *   "ts"            int2[F]   intersections {curve id, t bits | flags}  (ras_cut.h:837-1241)
+ *   "frag_yx"       int32[F]  raw fragment cell key, in ts order          (.cu:255-262)
+ *   "frag_pid"      int32[F]  path id | winding change << 30              (.cu:414)
+ *   "frag_wn"       int8[F]   scan winding number                         (.cu:264-278)
+ *   "frag_mask"     int32[4F] coverage mask, 4 words per fragment         (.cu:330-371)
+ *   "merged_mask"   int32[4M] resolved per-sample masks, draw-list order  (.cu:554-786)
+ *   "draw_list"     int4[M+S] {y<<16|x, width, fill, mask index+1}        (.cu:789-868)
+ *   "image"         u8[W*H*4] the output image
+ * `nbytes` receives the size; a NULL or too-small destination only queries it. */
+int sl_debug_enable(sl_handle *h, int enable);
+int sl_debug_dump(sl_handle *h, const char *name, void *host_dst, size_t capacity,
+                  size_t *nbytes);
+
+/* ---- seeded synthetic scenes ------------------------------------------------------------
+ * Host-only generators for the benchmark workloads (splitmix64, one substream per
+ * path).  kind: 1 = closed chains of cubics with control points uniform in the canvas;
+ * 2 = mixed line/quad/cubic paths with log-uniform bboxes, star polygons, alternating
+ * fill rules; 3 = glyph-like contours of quadratics with reversed holes.
+ * `min_segs`/`max_segs` <= 0 select the kind's defaults.  The scene stays owned by the
+ * generator object until sl_scene_free. */
+typedef struct sl_scene_obj sl_scene_obj;
+int sl_scene_generate(int kind, int width, int height, int n_paths, uint64_t seed,
+                      int min_segs, int max_segs, sl_scene_obj **out);
+int sl_scene_view(const sl_scene_obj *obj, sl_scene *out);
+int sl_scene_free(sl_scene_obj *obj);
+
+/* Library version string. */
+const char *sl_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCANLINE_MI355X_H */
