@@ -1,0 +1,668 @@
+// sl_api.cpp -- the C ABI (include/scanline.h): handle state, scene upload, per-frame
+// orchestration of the HIP kernels, timing and parity dumps.
+//
+// The frame follows rasterizeImpl<2> (ras_cut_mask_comb_scanline.cu:879-1453):
+//   transform+cull -> monotone split + fragment count -> scan -> intersections ->
+//   (path,row) buckets -> fragment count / scatter -> per-bucket resolve (sort by x,
+//   winding scan, stencil, spans) -> row order -> compute compositor.
+// Like the reference it reads two counts back to the host per frame (F after the curve
+// scan, the bucket total after the path scan) to size its buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/scanline.h"
+#include "sl_internal.h"
+#include "sl_tables.h"
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Stage {
+    const char *name;
+};
+const char *kStageNames[] = {"transform_cull", "monotone_count", "intersect", "bucket_setup",
+                             "fragments",      "resolve",        "order_rows", "composite"};
+constexpr int kStages = 8;
+
+}  // namespace
+
+struct sl_handle {
+    int device = 0;
+    int spp = 32;
+    int ms = 1;
+    int srgb = 1;
+    float m[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};  // column-major
+    int W = 0, H = 0;
+    bool has_scene = false;
+    uint32_t P = 0, C = 0, V = 0;
+    std::string err;
+    sl_counts counts{};
+    int64_t pixel_count = 0;
+    bool timing = false;
+    double stage_ms[kStages] = {0};
+    bool debug = false;
+
+    // scene (device)
+    DevBuf vtx, vpath, cvpos, ctype, cpath, cw, crev, frule, finfo;
+    // tables (device)
+    DevBuf ptable[3];
+    DevBuf ctab[2];  // [srgb] : dec_c, thr_c, dec_a, thr_a, lut_c, lut_a packed
+    // per-frame arena
+    DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
+        b_boff, b_bwsum, b_bcarry, b_bcursor, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_bicnt, b_bmcnt,
+        b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
+        b_rowbeg, b_rowend, b_image, b_scan_tmp;
+    // debug host copies
+    std::vector<int32_t> d_ts, d_yx, d_pid, d_draw;
+    std::vector<int8_t> d_wn;
+    std::vector<uint32_t> d_mask, d_mmask, d_ptable;
+    std::vector<uint8_t> d_image;
+    std::vector<uint32_t> d_binfo, d_bicnt, d_bext, d_boff, d_rowkeys, d_rowvals, d_rowbeg, d_rowend;
+    hipEvent_t ev[kStages + 1] = {};
+    bool events_ready = false;
+    int64_t NB = 0;
+    uint32_t *nz_vals_sorted = nullptr;
+};
+
+namespace {
+
+int fail(sl_handle *h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                                  \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess) return fail((h), SL_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+int ensure(sl_handle *h, DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return SL_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) return fail(h, SL_E_HIP, "hipFree failed: %s", hipGetErrorString(e));
+    }
+    size_t want = bytes + bytes / 4;  // headroom for frame-to-frame growth
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        return fail(h, SL_E_OOM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    b.cap = want;
+    return SL_OK;
+}
+
+void release(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+template <class T> T *P_(DevBuf &b) { return (T *)b.p; }
+
+int upload(sl_handle *h, DevBuf &b, const void *src, size_t bytes) {
+    int rc = ensure(h, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIPCHK(h, hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return SL_OK;
+}
+
+int words_for(int spp) { return spp == 8 ? 1 : (spp == 16 ? 2 : 4); }
+int spp_slot(int spp) { return spp == 8 ? 0 : (spp == 16 ? 1 : 2); }
+
+// colour table block layout in one device buffer
+struct ColorBlock {
+    float dec_c[256], thr_c[256], dec_a[256], thr_a[256];
+    uint8_t lut_c[4096], lut_a[4096];
+};
+
+int ensure_tables(sl_handle *h) {
+    DevBuf &pt = h->ptable[spp_slot(h->spp)];
+    if (!pt.p) {
+        std::vector<uint32_t> tab;
+        sl::build_pixel_table(h->spp, tab);
+        int rc = upload(h, pt, tab.data(), tab.size() * 4);
+        if (rc) return rc;
+    }
+    DevBuf &ct = h->ctab[h->srgb ? 1 : 0];
+    if (!ct.p) {
+        sl::ColorTables c, a;
+        if (!sl::build_color_tables(h->srgb, c) || !sl::build_color_tables(0, a))
+            return fail(h, SL_E_UNSUPPORTED, "colour encode table has two thresholds in one LUT bucket");
+        ColorBlock blk;
+        std::memcpy(blk.dec_c, c.dec, sizeof blk.dec_c);
+        std::memcpy(blk.thr_c, c.thr, sizeof blk.thr_c);
+        std::memcpy(blk.lut_c, c.lut, sizeof blk.lut_c);
+        std::memcpy(blk.dec_a, a.dec, sizeof blk.dec_a);
+        std::memcpy(blk.thr_a, a.thr, sizeof blk.thr_a);
+        std::memcpy(blk.lut_a, a.lut, sizeof blk.lut_a);
+        int rc = upload(h, ct, &blk, sizeof blk);
+        if (rc) return rc;
+    }
+    return SL_OK;
+}
+
+sl::Tables tables_view(sl_handle *h) {
+    sl::Tables t;
+    t.spp = h->spp;
+    t.words = words_for(h->spp);
+    t.pixel = P_<uint32_t>(h->ptable[spp_slot(h->spp)]);
+    ColorBlock *cb = P_<ColorBlock>(h->ctab[h->srgb ? 1 : 0]);
+    t.dec_c = cb->dec_c; t.thr_c = cb->thr_c; t.lut_c = cb->lut_c;
+    t.dec_a = cb->dec_a; t.thr_a = cb->thr_a; t.lut_a = cb->lut_a;
+    return t;
+}
+
+sl::SceneDev scene_view(sl_handle *h) {
+    sl::SceneDev s;
+    s.P = h->P; s.C = h->C; s.V = h->V;
+    s.vtx = P_<float2>(h->vtx); s.vpath = P_<uint32_t>(h->vpath);
+    s.cvpos = P_<uint32_t>(h->cvpos); s.ctype = P_<uint8_t>(h->ctype); s.cpath = P_<uint32_t>(h->cpath);
+    s.cw = P_<float>(h->cw); s.crev = P_<uint8_t>(h->crev);
+    s.frule = P_<uint8_t>(h->frule); s.finfo = P_<uint32_t>(h->finfo);
+    return s;
+}
+
+int check_launch(sl_handle *h, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(h, SL_E_HIP, "kernel launch (%s) failed: %s", what, hipGetErrorString(e));
+    return SL_OK;
+}
+
+uint32_t read_u32(sl_handle *h, const uint32_t *dptr, hipStream_t s, int &rc) {
+    uint32_t v = 0;
+    hipError_t e = hipMemcpyAsync(&v, dptr, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = fail(h, SL_E_HIP, "count read-back failed: %s", hipGetErrorString(e));
+    return v;
+}
+
+int render(sl_handle *h, hipStream_t s) {
+    int rc;
+    if (!h->has_scene) return fail(h, SL_E_STATE, "rasterize called before add_vg");
+    if (h->W <= 0 || h->H <= 0) return fail(h, SL_E_STATE, "output size not set (resize_output_buffer)");
+    if ((rc = ensure_tables(h))) return rc;
+    const sl::SceneDev sc = scene_view(h);
+    const sl::Tables tb = tables_view(h);
+    const int NW = tb.words;
+    const uint32_t P = h->P, C = h->C, V = h->V;
+
+    sl::Frame f{};
+    f.W = h->W; f.H = h->H;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) f.m[4 * i + j] = h->m[4 * j + i];
+    f.srgb = h->srgb; f.ms = h->ms;
+    f.pitch = h->W * 4;
+
+    size_t scan_need = sl::scan_tmp_elems_needed((size_t)std::max(C, P) + 1);
+    if ((rc = ensure(h, h->b_tvtx, (size_t)V * 8)) || (rc = ensure(h, h->b_pflags, (size_t)P * 4)) ||
+        (rc = ensure(h, h->b_prow, (size_t)P * 8)) || (rc = ensure(h, h->b_ccuts, (size_t)C * 16)) ||
+        (rc = ensure(h, h->b_cncut, (size_t)C)) || (rc = ensure(h, h->b_cfrag, ((size_t)C + 1) * 4)) ||
+        (rc = ensure(h, h->b_coff, ((size_t)C + 1) * 4)) || (rc = ensure(h, h->b_pnb, ((size_t)P + 1) * 4)) ||
+        (rc = ensure(h, h->b_pbase, ((size_t)P + 1) * 4)) || (rc = ensure(h, h->b_small, 64)) ||
+        (rc = ensure(h, h->b_image, (size_t)h->H * f.pitch)) || (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
+        return rc;
+    f.tvtx = P_<float2>(h->b_tvtx); f.pflags = P_<uint32_t>(h->b_pflags); f.prow = P_<int2>(h->b_prow);
+    f.ccuts = P_<float4>(h->b_ccuts); f.cncut = P_<uint8_t>(h->b_cncut); f.cfrag = P_<uint32_t>(h->b_cfrag);
+    f.coff = P_<uint32_t>(h->b_coff); f.pnb = P_<uint32_t>(h->b_pnb); f.pbase = P_<uint32_t>(h->b_pbase);
+    f.image = P_<uint8_t>(h->b_image);
+    f.big_count = P_<uint32_t>(h->b_small);
+    f.err_flag = P_<uint32_t>(h->b_small) + 1;
+    f.scan_tmp = P_<uint32_t>(h->b_scan_tmp);
+
+    auto mark = [&](int i) {
+        if (h->timing) (void)hipEventRecord(h->ev[i], s);
+    };
+    mark(0);
+    sl::launch_fill_u32(f.err_flag, 0u, 1, s);
+    sl::launch_path_init(sc, f, s);
+    sl::launch_transform(sc, f, s);
+    if ((rc = check_launch(h, "transform"))) return rc;
+    mark(1);
+    sl::launch_monotone(sc, f, s);
+    sl::scan_exclusive_u32(f.cfrag, f.coff, C, f.scan_tmp, s);
+    if ((rc = check_launch(h, "monotone"))) return rc;
+    rc = SL_OK;
+    const uint32_t F = read_u32(h, f.coff + C, s, rc);
+    if (rc) return rc;
+    f.F = F;
+    h->counts.path = P; h->counts.curve = C; h->counts.vertex = V;
+    h->counts.curve_fragment = F;
+    h->counts.merged_fragment = 0;
+    h->counts.span = 0;
+    h->pixel_count = 0;
+    h->NB = 0;
+    if (F == 0) {
+        // nothing to draw: the white clear of the reference's output pass
+        sl::launch_fill_u8(f.image, 255, (size_t)h->H * f.pitch, s);
+        for (int i = 2; i <= kStages; ++i) mark(i);
+        if ((rc = check_launch(h, "clear"))) return rc;
+        HIPCHK(h, hipStreamSynchronize(s));
+        return SL_OK;
+    }
+    if ((rc = ensure(h, h->b_ts, (size_t)F * 8))) return rc;
+    f.ts = P_<int2>(h->b_ts);
+    mark(2);
+    sl::launch_intersect(sc, f, s);
+    if ((rc = check_launch(h, "intersect"))) return rc;
+    mark(3);
+    sl::launch_path_buckets(sc, f, s);
+    sl::scan_exclusive_u32(f.pnb, f.pbase, P, f.scan_tmp, s);
+    const uint32_t NB = read_u32(h, f.pbase + P, s, rc);
+    if (rc) return rc;
+    f.NB = NB;
+    h->NB = NB;
+    const size_t R = (size_t)(h->H + 1) / 2;
+    const size_t rs_tiles = ((size_t)NB + 2047) / 2048;
+    scan_need = sl::scan_tmp_elems_needed(std::max<size_t>({(size_t)C + 1, (size_t)P + 1, (size_t)NB + 1, 256 * rs_tiles + 1}));
+    if ((rc = ensure(h, h->b_binfo, (size_t)NB * 8)) || (rc = ensure(h, h->b_bcount, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure(h, h->b_boff, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bwsum, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure(h, h->b_bcarry, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bcursor, (size_t)NB * 4)) ||
+        (rc = ensure(h, h->b_rec_meta, (size_t)F * 4)) || (rc = ensure(h, h->b_rec_mask, (size_t)F * 4 * NW)) ||
+        (rc = ensure(h, h->b_scratch, (size_t)F * 8)) || (rc = ensure(h, h->b_items, (size_t)F * 8)) ||
+        (rc = ensure(h, h->b_imask, (size_t)F * 8 * NW)) || (rc = ensure(h, h->b_bicnt, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure(h, h->b_bmcnt, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bext, (size_t)NB * 4)) ||
+        (rc = ensure(h, h->b_big_list, (size_t)NB * 4)) || (rc = ensure(h, h->b_nz_flag, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure(h, h->b_nz_off, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_nzk0, (size_t)NB * 4)) ||
+        (rc = ensure(h, h->b_nzk1, (size_t)NB * 4)) || (rc = ensure(h, h->b_nzv0, (size_t)NB * 4)) ||
+        (rc = ensure(h, h->b_nzv1, (size_t)NB * 4)) || (rc = ensure(h, h->b_rs_hist, 256 * rs_tiles * 4 + 16)) ||
+        (rc = ensure(h, h->b_rs_hoff, (256 * rs_tiles + 1) * 4 + 16)) || (rc = ensure(h, h->b_rowbeg, R * 4)) ||
+        (rc = ensure(h, h->b_rowend, R * 4)) || (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
+        return rc;
+    f.binfo = P_<int2>(h->b_binfo); f.bcount = P_<uint32_t>(h->b_bcount); f.boff = P_<uint32_t>(h->b_boff);
+    f.bwsum = P_<uint32_t>(h->b_bwsum); f.bcarry = P_<uint32_t>(h->b_bcarry); f.bcursor = P_<uint32_t>(h->b_bcursor);
+    f.rec_meta = P_<uint32_t>(h->b_rec_meta); f.rec_mask = P_<uint32_t>(h->b_rec_mask);
+    f.scratch_keys = P_<uint32_t>(h->b_scratch); f.items = P_<uint32_t>(h->b_items); f.imask = P_<uint32_t>(h->b_imask);
+    f.bicnt = P_<uint32_t>(h->b_bicnt); f.bmcnt = P_<uint32_t>(h->b_bmcnt); f.bext = P_<uint32_t>(h->b_bext);
+    f.big_list = P_<uint32_t>(h->b_big_list); f.nz_flag = P_<uint32_t>(h->b_nz_flag); f.nz_off = P_<uint32_t>(h->b_nz_off);
+    f.nz_keys[0] = P_<uint32_t>(h->b_nzk0); f.nz_keys[1] = P_<uint32_t>(h->b_nzk1);
+    f.nz_vals[0] = P_<uint32_t>(h->b_nzv0); f.nz_vals[1] = P_<uint32_t>(h->b_nzv1);
+    f.rs_hist = P_<uint32_t>(h->b_rs_hist); f.rs_hoff = P_<uint32_t>(h->b_rs_hoff);
+    f.rowbeg = P_<uint32_t>(h->b_rowbeg); f.rowend = P_<uint32_t>(h->b_rowend);
+    f.scan_tmp = P_<uint32_t>(h->b_scan_tmp);
+
+    sl::launch_bucket_info(sc, f, s);
+    if ((rc = check_launch(h, "bucket_info"))) return rc;
+    mark(4);
+    sl::launch_frag_count(sc, tb, f, s);
+    sl::scan_exclusive_u32(f.bcount, f.boff, NB, f.scan_tmp, s);
+    sl::scan_exclusive_u32(f.bwsum, f.bcarry, NB, f.scan_tmp, s);
+    sl::launch_frag_scatter(sc, tb, f, s);
+    if ((rc = check_launch(h, "fragments"))) return rc;
+    mark(5);
+    sl::launch_resolve(sc, tb, f, s);
+    if ((rc = check_launch(h, "resolve"))) return rc;
+    mark(6);
+    sl::launch_order_rows(sc, f, s);
+    if ((rc = check_launch(h, "order_rows"))) return rc;
+    h->nz_vals_sorted = f.nz_vals[0];
+    mark(7);
+    sl::launch_composite(sc, tb, f, s);
+    if ((rc = check_launch(h, "composite"))) return rc;
+    mark(8);
+
+    // element counts (M, S) and the error flag
+    unsigned long long *red = (unsigned long long *)(P_<uint32_t>(h->b_small) + 4);
+    sl::launch_fill_u32((uint32_t *)red, 0u, 4, s);
+    sl::launch_count_reduce(f.bmcnt, NB, red, s);
+    sl::launch_count_reduce(f.bicnt, NB, red + 1, s);
+    uint32_t small[8];
+    HIPCHK(h, hipMemcpyAsync(small, h->b_small.p, 32, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    const unsigned long long M = *(unsigned long long *)&small[4], MS = *(unsigned long long *)&small[6];
+    h->counts.merged_fragment = (int64_t)M;
+    h->counts.span = (int64_t)(MS - M);
+    if (small[1] & 1u) return fail(h, SL_E_HIP, "internal error: fragment row outside its path's bucket range");
+    if (small[1] & 2u) return fail(h, SL_E_UNSUPPORTED, "a (path, row) bucket holds more than 65536 fragments");
+
+    if (h->timing) {
+        for (int i = 0; i < kStages; ++i) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]);
+            h->stage_ms[i] = ms;
+        }
+    }
+
+    if (h->debug) {
+        // parity dumps in the reference's layouts
+        h->d_ts.resize((size_t)F * 2);
+        HIPCHK(h, hipMemcpy(h->d_ts.data(), f.ts, (size_t)F * 8, hipMemcpyDeviceToHost));
+        DevBuf dyx, dpid, dwn, dmask, dmoff, dioff, ddraw, dmm;
+        int r2 = ensure(h, dyx, (size_t)F * 4);
+        if (!r2) r2 = ensure(h, dpid, (size_t)F * 4);
+        if (!r2) r2 = ensure(h, dwn, (size_t)F);
+        if (!r2) r2 = ensure(h, dmask, (size_t)F * 16);
+        if (!r2) r2 = ensure(h, dmoff, ((size_t)NB + 1) * 4);
+        if (!r2) r2 = ensure(h, dioff, ((size_t)NB + 1) * 4);
+        if (!r2) r2 = ensure(h, ddraw, (size_t)std::max<unsigned long long>(MS, 1) * 16);
+        if (!r2) r2 = ensure(h, dmm, (size_t)std::max<unsigned long long>(M, 1) * 16);
+        if (!r2) {
+            sl::launch_debug_fragments(sc, tb, f, P_<int32_t>(dyx), P_<int32_t>(dpid), P_<int8_t>(dwn),
+                                       P_<uint32_t>(dmask), s);
+            sl::scan_exclusive_u32(f.bmcnt, P_<uint32_t>(dmoff), NB, f.scan_tmp, s);
+            sl::scan_exclusive_u32(f.bicnt, P_<uint32_t>(dioff), NB, f.scan_tmp, s);
+            sl::launch_debug_draw_list(sc, tb, f, P_<uint32_t>(dmoff), P_<uint32_t>(dioff), P_<int4>(ddraw),
+                                       P_<uint32_t>(dmm), s);
+            r2 = check_launch(h, "debug");
+        }
+        if (!r2) {
+            h->d_yx.resize(F); h->d_pid.resize(F); h->d_wn.resize(F); h->d_mask.resize((size_t)F * 4);
+            h->d_draw.resize((size_t)MS * 4); h->d_mmask.resize((size_t)M * 4);
+            hipError_t e = hipMemcpy(h->d_yx.data(), dyx.p, (size_t)F * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(h->d_pid.data(), dpid.p, (size_t)F * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(h->d_wn.data(), dwn.p, (size_t)F, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(h->d_mask.data(), dmask.p, (size_t)F * 16, hipMemcpyDeviceToHost);
+            if (e == hipSuccess && MS) e = hipMemcpy(h->d_draw.data(), ddraw.p, (size_t)MS * 16, hipMemcpyDeviceToHost);
+            if (e == hipSuccess && M) e = hipMemcpy(h->d_mmask.data(), dmm.p, (size_t)M * 16, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) r2 = fail(h, SL_E_HIP, "debug copy failed: %s", hipGetErrorString(e));
+        }
+        if (!r2) {
+            // internal bucket / row-order arrays for diagnosing the compositor
+            uint32_t NZ = 0;
+            hipError_t e = hipMemcpy(&NZ, f.nz_off + NB, 4, hipMemcpyDeviceToHost);
+            h->d_binfo.resize((size_t)NB * 2); h->d_bicnt.resize(NB); h->d_bext.resize(NB);
+            h->d_boff.resize((size_t)NB + 1); h->d_rowkeys.resize(NZ); h->d_rowvals.resize(NZ);
+            h->d_rowbeg.resize(R); h->d_rowend.resize(R);
+            if (e == hipSuccess) e = hipMemcpy(h->d_binfo.data(), f.binfo, (size_t)NB * 8, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(h->d_bicnt.data(), f.bicnt, (size_t)NB * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(h->d_bext.data(), f.bext, (size_t)NB * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(h->d_boff.data(), f.boff, ((size_t)NB + 1) * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess && NZ) e = hipMemcpy(h->d_rowkeys.data(), f.nz_keys[0], (size_t)NZ * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess && NZ) e = hipMemcpy(h->d_rowvals.data(), f.nz_vals[0], (size_t)NZ * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(h->d_rowbeg.data(), f.rowbeg, R * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(h->d_rowend.data(), f.rowend, R * 4, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) r2 = fail(h, SL_E_HIP, "debug copy failed: %s", hipGetErrorString(e));
+        }
+        int64_t pc = 0;
+        for (size_t k = 0; k < (size_t)MS; ++k) pc += 2 * (int64_t)h->d_draw[4 * k + 1];
+        h->pixel_count = pc;
+        release(dyx); release(dpid); release(dwn); release(dmask); release(dmoff); release(dioff);
+        release(ddraw); release(dmm);
+        if (r2) return r2;
+    }
+    return SL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sl_version(void) { return "scanline-mi355x 0.1 (gfx950)"; }
+
+int sl_create(int device, sl_handle **out) {
+    if (!out) return SL_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return SL_E_NODEVICE;
+    }
+    if (device < 0) device = 0;
+    if (device >= n) return SL_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SL_E_HIP;
+    sl_handle *h = new (std::nothrow) sl_handle;
+    if (!h) return SL_E_OOM;
+    h->device = device;
+    for (int i = 0; i <= kStages; ++i) {
+        if (hipEventCreate(&h->ev[i]) != hipSuccess) {
+            delete h;
+            return SL_E_HIP;
+        }
+    }
+    h->events_ready = true;
+    *out = h;
+    return SL_OK;
+}
+
+int sl_destroy(sl_handle *h) {
+    if (!h) return SL_OK;
+    (void)hipSetDevice(h->device);
+    (void)hipDeviceSynchronize();
+    DevBuf *bufs[] = {&h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->frule, &h->finfo,
+                      &h->ptable[0], &h->ptable[1], &h->ptable[2], &h->ctab[0], &h->ctab[1], &h->b_tvtx, &h->b_pflags,
+                      &h->b_prow, &h->b_ccuts, &h->b_cncut, &h->b_cfrag, &h->b_coff, &h->b_ts, &h->b_pnb, &h->b_pbase,
+                      &h->b_binfo, &h->b_bcount, &h->b_boff, &h->b_bwsum, &h->b_bcarry, &h->b_bcursor, &h->b_rec_meta,
+                      &h->b_rec_mask, &h->b_scratch, &h->b_items, &h->b_imask, &h->b_bicnt, &h->b_bmcnt, &h->b_bext,
+                      &h->b_big_list, &h->b_small, &h->b_nz_flag, &h->b_nz_off, &h->b_nzk0, &h->b_nzk1, &h->b_nzv0,
+                      &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp};
+    for (DevBuf *b : bufs) release(*b);
+    if (h->events_ready)
+        for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(h->ev[i]);
+    delete h;
+    return SL_OK;
+}
+
+int sl_set_samples(sl_handle *h, int samples) {
+    if (!h) return SL_E_ARG;
+    if (samples != 8 && samples != 16 && samples != 32)
+        return fail(h, SL_E_UNSUPPORTED, "samples must be 8, 16 or 32 (got %d)", samples);
+    h->spp = samples;
+    return SL_OK;
+}
+
+int sl_set_multisample_output(sl_handle *h, int enable) {
+    if (!h) return SL_E_ARG;
+    h->ms = enable ? 1 : 0;
+    return SL_OK;
+}
+
+int sl_enable_srgb_correction(sl_handle *h, int enable) {
+    if (!h) return SL_E_ARG;
+    h->srgb = enable ? 1 : 0;
+    return SL_OK;
+}
+
+int sl_set_input_transform(sl_handle *h, const float m[16]) {
+    if (!h || !m) return SL_E_ARG;
+    std::memcpy(h->m, m, sizeof h->m);
+    return SL_OK;
+}
+
+int sl_resize_output_buffer(sl_handle *h, int width, int height) {
+    if (!h) return SL_E_ARG;
+    if (width <= 0 || height <= 0 || width > 32766 || height > 32766)
+        return fail(h, SL_E_ARG, "output size %dx%d outside 1..32766", width, height);
+    h->W = width;
+    h->H = height;
+    return SL_OK;
+}
+
+int sl_add_vg(sl_handle *h, const sl_scene *s) {
+    if (!h) return SL_E_ARG;
+    if (!s) return fail(h, SL_E_ARG, "scene is NULL");
+    const uint32_t P = s->n_paths, C = s->n_curves, V = s->n_vertices;
+    if ((C && (!s->curve_vertex_pos || !s->curve_type || !s->curve_path_id)) || (V && !s->vertex) ||
+        (P && (!s->fill_rule || !s->fill_color)))
+        return fail(h, SL_E_ARG, "scene array pointer is NULL");
+    if (P == 0 && C != 0) return fail(h, SL_E_ARG, "curves without paths");
+    std::vector<float> vtx(s->vertex, s->vertex + 2 * (size_t)V);
+    std::vector<uint32_t> vpath(V, 0xFFFFFFFFu);
+    std::vector<uint8_t> crev(C, 0);
+    std::vector<float> cw(C, -1.f);
+    uint32_t last_path = 0;
+    for (uint32_t c = 0; c < C; ++c) {
+        const uint8_t t = s->curve_type[c];
+        if (t != SL_CURVE_LINEAR && t != SL_CURVE_QUADRATIC && t != SL_CURVE_CUBIC && t != SL_CURVE_RATIONAL)
+            return fail(h, SL_E_ARG, "curve %u has invalid type 0x%x", c, t);
+        const uint32_t p = s->curve_path_id[c];
+        if (p >= P) return fail(h, SL_E_ARG, "curve %u has path id %u >= n_paths %u", c, p, P);
+        if (p < last_path) return fail(h, SL_E_ARG, "curve %u: curve_path_id must be non-decreasing", c);
+        last_path = p;
+        const uint32_t vp = s->curve_vertex_pos[c];
+        const int nv = t & 7;
+        if ((uint64_t)vp + nv > V) return fail(h, SL_E_ARG, "curve %u vertices [%u, %u) exceed n_vertices %u", c, vp, vp + nv, V);
+        for (int k = 0; k < nv; ++k) {
+            if (vpath[vp + k] != 0xFFFFFFFFu) return fail(h, SL_E_ARG, "vertex %u belongs to two curves", vp + k);
+            vpath[vp + k] = p;
+        }
+        if (t == SL_CURVE_RATIONAL) cw[c] = s->curve_arc_w ? s->curve_arc_w[c] : 1.f;
+        // unifyCurveVertexOrder (ras_base.cpp:523-569): order by float bit patterns
+        uint32_t ixf, ixl, iyf, iyl;
+        std::memcpy(&ixf, &vtx[2 * (size_t)vp], 4);
+        std::memcpy(&iyf, &vtx[2 * (size_t)vp + 1], 4);
+        std::memcpy(&ixl, &vtx[2 * (size_t)(vp + nv - 1)], 4);
+        std::memcpy(&iyl, &vtx[2 * (size_t)(vp + nv - 1) + 1], 4);
+        if (!(iyf < iyl || (iyf == iyl && ixf < ixl))) {
+            crev[c] = 1;
+            for (int k = 0; k < nv / 2; ++k) {
+                std::swap(vtx[2 * (size_t)(vp + k)], vtx[2 * (size_t)(vp + nv - 1 - k)]);
+                std::swap(vtx[2 * (size_t)(vp + k) + 1], vtx[2 * (size_t)(vp + nv - 1 - k) + 1]);
+            }
+        }
+    }
+    for (uint32_t i = 0; i < V; ++i)
+        if (vpath[i] == 0xFFFFFFFFu) vpath[i] = 0;  // vertices of no curve: harmless, never read
+    std::vector<uint32_t> finfo(P);
+    for (uint32_t p = 0; p < P; ++p) {
+        const uint32_t c = s->fill_color[p];
+        finfo[p] = (c & 0xFF000000u) ? c : 0u;  // ras_base.cpp:147-151
+    }
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    int rc;
+    if ((rc = upload(h, h->vtx, vtx.data(), vtx.size() * 4)) || (rc = upload(h, h->vpath, vpath.data(), (size_t)V * 4)) ||
+        (rc = upload(h, h->cvpos, s->curve_vertex_pos, (size_t)C * 4)) || (rc = upload(h, h->ctype, s->curve_type, C)) ||
+        (rc = upload(h, h->cpath, s->curve_path_id, (size_t)C * 4)) || (rc = upload(h, h->cw, cw.data(), (size_t)C * 4)) ||
+        (rc = upload(h, h->crev, crev.data(), C)) || (rc = upload(h, h->frule, s->fill_rule, P)) ||
+        (rc = upload(h, h->finfo, finfo.data(), (size_t)P * 4)))
+        return rc;
+    h->P = P; h->C = C; h->V = (P == 0) ? 0 : V;
+    h->has_scene = true;
+    return SL_OK;
+}
+
+int sl_clear(sl_handle *h) {
+    if (!h) return SL_E_ARG;
+    h->has_scene = false;
+    h->P = h->C = h->V = 0;
+    return SL_OK;
+}
+
+int sl_rasterize(sl_handle *h, void *stream) {
+    if (!h) return SL_E_ARG;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    return render(h, (hipStream_t)stream);
+}
+
+int sl_output_device_ptr(sl_handle *h, void **dev_ptr, int *pitch_bytes) {
+    if (!h || !dev_ptr || !pitch_bytes) return SL_E_ARG;
+    if (!h->b_image.p) return fail(h, SL_E_STATE, "no image rendered yet");
+    *dev_ptr = h->b_image.p;
+    *pitch_bytes = h->W * 4;
+    return SL_OK;
+}
+
+int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags) {
+    if (!h || !host_rgba) return SL_E_ARG;
+    if (!h->b_image.p) return fail(h, SL_E_STATE, "no image rendered yet");
+    const size_t row = (size_t)h->W * 4;
+    HIPCHK(h, hipMemcpy(host_rgba, h->b_image.p, row * h->H, hipMemcpyDeviceToHost));
+    if (flags & 1) {
+        std::vector<uint8_t> tmp(row);
+        for (int y = 0; y < h->H / 2; ++y) {
+            uint8_t *a = host_rgba + row * y, *b = host_rgba + row * (h->H - 1 - y);
+            std::memcpy(tmp.data(), a, row);
+            std::memcpy(a, b, row);
+            std::memcpy(b, tmp.data(), row);
+        }
+        for (size_t i = 3; i < row * h->H; i += 4) host_rgba[i] = 255;
+    }
+    return SL_OK;
+}
+
+int sl_element_number(sl_handle *h, sl_counts *out) {
+    if (!h || !out) return SL_E_ARG;
+    *out = h->counts;
+    return SL_OK;
+}
+
+int sl_enable_step_timing(sl_handle *h, int enable) {
+    if (!h) return SL_E_ARG;
+    h->timing = enable != 0;
+    return SL_OK;
+}
+
+int sl_step_timing(sl_handle *h, double *ms, int capacity, int *n_stages) {
+    if (!h || !n_stages) return SL_E_ARG;
+    *n_stages = kStages;
+    if (ms)
+        for (int i = 0; i < kStages && i < capacity; ++i) ms[i] = h->stage_ms[i];
+    return SL_OK;
+}
+
+const char *sl_step_name(int stage) { return (stage >= 0 && stage < kStages) ? kStageNames[stage] : ""; }
+
+int sl_pixel_count(sl_handle *h, int64_t *out) {
+    if (!h || !out) return SL_E_ARG;
+    if (!h->debug) return fail(h, SL_E_STATE, "pixel count needs sl_debug_enable (it reads the draw list back)");
+    *out = h->pixel_count;
+    return SL_OK;
+}
+
+const char *sl_last_error(sl_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int sl_debug_enable(sl_handle *h, int enable) {
+    if (!h) return SL_E_ARG;
+    h->debug = enable != 0;
+    return SL_OK;
+}
+
+int sl_debug_dump(sl_handle *h, const char *name, void *host_dst, size_t capacity, size_t *nbytes) {
+    if (!h || !name || !nbytes) return SL_E_ARG;
+    const void *src = nullptr;
+    size_t n = 0;
+    std::string k(name);
+    if (k == "image") {
+        n = (size_t)h->W * h->H * 4;
+        if (!h->b_image.p) return fail(h, SL_E_STATE, "no image");
+        *nbytes = n;
+        if (host_dst && capacity >= n) HIPCHK(h, hipMemcpy(host_dst, h->b_image.p, n, hipMemcpyDeviceToHost));
+        return SL_OK;
+    }
+    if (k == "pixel_table") {
+        sl::build_pixel_table(h->spp, h->d_ptable);
+        src = h->d_ptable.data(); n = h->d_ptable.size() * 4;
+    } else if (!h->debug) {
+        return fail(h, SL_E_STATE, "debug dumps need sl_debug_enable before sl_rasterize");
+    } else if (k == "ts") { src = h->d_ts.data(); n = h->d_ts.size() * 4; }
+    else if (k == "frag_yx") { src = h->d_yx.data(); n = h->d_yx.size() * 4; }
+    else if (k == "frag_pid") { src = h->d_pid.data(); n = h->d_pid.size() * 4; }
+    else if (k == "frag_wn") { src = h->d_wn.data(); n = h->d_wn.size(); }
+    else if (k == "frag_mask") { src = h->d_mask.data(); n = h->d_mask.size() * 4; }
+    else if (k == "merged_mask") { src = h->d_mmask.data(); n = h->d_mmask.size() * 4; }
+    else if (k == "draw_list") { src = h->d_draw.data(); n = h->d_draw.size() * 4; }
+    else if (k == "bucket_info") { src = h->d_binfo.data(); n = h->d_binfo.size() * 4; }
+    else if (k == "bucket_items") { src = h->d_bicnt.data(); n = h->d_bicnt.size() * 4; }
+    else if (k == "bucket_ext") { src = h->d_bext.data(); n = h->d_bext.size() * 4; }
+    else if (k == "bucket_off") { src = h->d_boff.data(); n = h->d_boff.size() * 4; }
+    else if (k == "row_keys") { src = h->d_rowkeys.data(); n = h->d_rowkeys.size() * 4; }
+    else if (k == "row_list") { src = h->d_rowvals.data(); n = h->d_rowvals.size() * 4; }
+    else if (k == "row_beg") { src = h->d_rowbeg.data(); n = h->d_rowbeg.size() * 4; }
+    else if (k == "row_end") { src = h->d_rowend.data(); n = h->d_rowend.size() * 4; }
+    else return fail(h, SL_E_ARG, "unknown dump '%s'", name);
+    *nbytes = n;
+    if (host_dst && capacity >= n && n) std::memcpy(host_dst, src, n);
+    return SL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// sl_internal.h -- declarations shared by the host orchestration (sl_api.cpp) and the
+// HIP kernels (sl_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sl {
+
+// Per-frame device views handed to the kernel launchers.  Every pointer is a device
+// pointer owned by the handle's arena.
+struct SceneDev {
+    uint32_t P, C, V;
+    const float2 *vtx;       // [V] vertices in unified (orientation-fixed) order
+    const uint32_t *vpath;   // [V] owning path of each vertex
+    const uint32_t *cvpos;   // [C]
+    const uint8_t *ctype;    // [C]
+    const uint32_t *cpath;   // [C]
+    const float *cw;         // [C] rational weight (else -1)
+    const uint8_t *crev;     // [C] 1 = vertex order was reversed at upload
+    const uint8_t *frule;    // [P]
+    const uint32_t *finfo;   // [P] fill_info (RGBA8, alpha 0 -> 0)
+};
+
+struct Tables {
+    int spp;                 // 8, 16 or 32
+    int words;               // mask words per fragment: 1, 2, 4
+    const uint32_t *pixel;   // [65536 * words] line -> sample mask table
+    const float *dec_c;      // [256] colour decode (sRGB or UNORM)
+    const float *thr_c;      // [256] colour encode thresholds
+    const uint8_t *lut_c;    // [4096] encode LUT
+    const float *dec_a;      // [256] alpha decode (UNORM)
+    const float *thr_a;      // [256]
+    const uint8_t *lut_a;    // [4096]
+};
+
+struct Frame {
+    int W, H;
+    float m[16];             // rows of the input transform, row-major
+    // path / curve level
+    float2 *tvtx;            // [V]
+    uint32_t *pflags;        // [P] 8 region bits
+    int2 *prow;              // [P] min / max valid cell row
+    float4 *ccuts;           // [C]
+    uint8_t *cncut;          // [C]
+    uint32_t *cfrag;         // [C+1]  -> exclusive offsets after scan
+    uint32_t *coff;          // [C+1]
+    // fragments
+    int64_t F;
+    int2 *ts;                // [F]
+    uint32_t *pnb;           // [P+1] buckets per path -> scanned to pbase
+    uint32_t *pbase;         // [P+1]
+    int64_t NB;
+    int2 *binfo;             // [NB] (path, row or -1)
+    uint32_t *bcount;        // [NB+1] -> boff
+    uint32_t *boff;          // [NB+1]
+    uint32_t *bwsum;         // [NB+1] -> bcarry
+    uint32_t *bcarry;        // [NB+1]
+    uint32_t *bcursor;       // [NB]
+    uint32_t *rec_meta;      // [F] (valid fragments only are written)
+    uint32_t *rec_mask;      // [F * words]
+    uint32_t *scratch_keys;  // [F] sort keys of large buckets
+    // resolve outputs, per bucket and per item slot (2 slots per valid fragment)
+    uint32_t *items;         // [2F]
+    uint32_t *imask;         // [2F * words]
+    uint32_t *bicnt;         // [NB+1]
+    uint32_t *bmcnt;         // [NB+1]
+    uint32_t *bext;          // [NB]
+    uint32_t *big_list;      // [NB] buckets with more than 64 fragments
+    uint32_t *big_count;     // [1]
+    uint32_t *err_flag;      // [1]
+    // compositing order
+    uint32_t *nz_flag;       // [NB+1]
+    uint32_t *nz_off;        // [NB+1]
+    uint32_t *nz_keys[2];    // [NB] row keys (ping-pong)
+    uint32_t *nz_vals[2];    // [NB] bucket ids (ping-pong)
+    uint32_t *rs_hist;       // radix histogram scratch
+    uint32_t *rs_hoff;
+    uint32_t *rowbeg;        // [R]
+    uint32_t *rowend;        // [R]
+    uint8_t *image;          // [H * pitch]
+    int pitch;
+    int srgb, ms;
+    // scan scratch
+    uint32_t *scan_tmp;
+    size_t scan_tmp_elems;
+};
+
+// ---- launchers (sl_kernels.hip) ----
+size_t scan_tmp_elems_needed(size_t n);
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s);
+
+void launch_path_init(const SceneDev &sc, Frame &f, hipStream_t s);
+void launch_transform(const SceneDev &sc, Frame &f, hipStream_t s);
+void launch_monotone(const SceneDev &sc, Frame &f, hipStream_t s);
+void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s);
+void launch_path_buckets(const SceneDev &sc, Frame &f, hipStream_t s);
+void launch_bucket_info(const SceneDev &sc, Frame &f, hipStream_t s);
+void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
+void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
+void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
+void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s);
+void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
+void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+
+// debug (parity dumps)
+void launch_debug_fragments(const SceneDev &sc, const Tables &t, Frame &f, int32_t *yx, int32_t *pid,
+                            int8_t *wn, uint32_t *mask4, hipStream_t s);
+void launch_debug_draw_list(const SceneDev &sc, const Tables &t, Frame &f, const uint32_t *moff,
+                            const uint32_t *ioff, int4 *draw, uint32_t *mmask4, hipStream_t s);
+void launch_count_reduce(const uint32_t *a, size_t n, unsigned long long *out, hipStream_t s);
+
+}  // namespace sl

@@ -1,0 +1,175 @@
+"""Scene containers in the reference's structure-of-arrays layout.
+
+``VGContainer`` mirrors the builder API of the reference's VGContainer
+(gpu-scanline/src/vg_container.h:69-140: addPath / addContour / addLinear / addQuadratic /
+addCubic / addRational, vertices stored per curve) and produces the arrays the C ABI's
+``sl_scene`` expects.  ``generate`` wraps the library's seeded synthetic scenes.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+
+class SceneArrays:
+    """Owns numpy arrays laid out as include/scanline.h's sl_scene."""
+
+    def __init__(self, vertex, curve_vertex_pos, curve_type, curve_path_id, curve_arc_w,
+                 fill_rule, fill_color):
+        self.vertex = np.ascontiguousarray(vertex, dtype=np.float32).reshape(-1, 2)
+        self.curve_vertex_pos = np.ascontiguousarray(curve_vertex_pos, dtype=np.uint32)
+        self.curve_type = np.ascontiguousarray(curve_type, dtype=np.uint8)
+        self.curve_path_id = np.ascontiguousarray(curve_path_id, dtype=np.uint32)
+        self.curve_arc_w = np.ascontiguousarray(curve_arc_w, dtype=np.float32)
+        self.fill_rule = np.ascontiguousarray(fill_rule, dtype=np.uint8)
+        self.fill_color = np.ascontiguousarray(fill_color, dtype=np.uint32)
+
+    @property
+    def n_paths(self):
+        return int(self.fill_rule.shape[0])
+
+    @property
+    def n_curves(self):
+        return int(self.curve_type.shape[0])
+
+    @property
+    def n_vertices(self):
+        return int(self.vertex.shape[0])
+
+    def as_c(self):
+        """sl_scene struct pointing into this object's arrays (keep self alive)."""
+        s = _abi.SlScene()
+        s.n_paths, s.n_curves, s.n_vertices = self.n_paths, self.n_curves, self.n_vertices
+        s.vertex = self.vertex.ctypes.data
+        s.curve_vertex_pos = self.curve_vertex_pos.ctypes.data
+        s.curve_type = self.curve_type.ctypes.data
+        s.curve_path_id = self.curve_path_id.ctypes.data
+        s.curve_arc_w = self.curve_arc_w.ctypes.data
+        s.fill_rule = self.fill_rule.ctypes.data
+        s.fill_color = self.fill_color.ctypes.data
+        return s
+
+    def oracle_args(self):
+        """Positional arrays in the order of the oracle's oracle_run."""
+        return (self.n_paths, self.n_curves, self.n_vertices, self.vertex, self.curve_vertex_pos,
+                self.curve_type, self.curve_path_id, self.curve_arc_w, self.fill_rule,
+                self.fill_color)
+
+
+def rgba(r, g, b, a=255):
+    """u8rgba packed little-endian (gpu-scanline/src/mochimazui/color.h:29-57)."""
+    return (int(r) & 255) | ((int(g) & 255) << 8) | ((int(b) & 255) << 16) | ((int(a) & 255) << 24)
+
+
+class VGContainer:
+    """Host-side scene builder with the reference's VGContainer method names."""
+
+    def __init__(self, width=0, height=0):
+        self.width, self.height = width, height
+        self._v, self._cpos, self._ctype, self._cpath, self._cw = [], [], [], [], []
+        self._rule, self._color = [], []
+
+    def pathNumber(self):
+        return len(self._rule)
+
+    def curveNumber(self):
+        return len(self._ctype)
+
+    def vertexNumber(self):
+        return len(self._v)
+
+    def addPath(self, f_rule=0, f_color=rgba(0, 0, 0)):
+        """addPath(f_rule, f_mode, f_color, f_index) (vg_container.h:120) for colour fills."""
+        self._rule.append(int(f_rule))
+        self._color.append(int(f_color))
+        return len(self._rule) - 1
+
+    def addContour(self):
+        """Contours only group curves in the reference; kept for API parity."""
+        return len(self._ctype)
+
+    def _curve(self, ctype, pts, w=-1.0):
+        if not self._rule:
+            raise ValueError("addPath before adding curves")
+        self._cpos.append(len(self._v))
+        self._ctype.append(ctype)
+        self._cpath.append(len(self._rule) - 1)
+        self._cw.append(w)
+        self._v.extend((float(x), float(y)) for x, y in pts)
+        return len(self._ctype) - 1
+
+    def addLinear(self, v0, v1):
+        return self._curve(_abi.CURVE_LINEAR, (v0, v1))
+
+    def addQuadratic(self, v0, v1, v2):
+        return self._curve(_abi.CURVE_QUADRATIC, (v0, v1, v2))
+
+    def addCubic(self, v0, v1, v2, v3):
+        return self._curve(_abi.CURVE_CUBIC, (v0, v1, v2, v3))
+
+    def addRational(self, v0, v1, v2, w):
+        return self._curve(_abi.CURVE_RATIONAL, (v0, v1, v2), float(w))
+
+    def addPolygon(self, pts):
+        """Closed polygon of lines (a convenience for tests)."""
+        for k in range(len(pts)):
+            self.addLinear(pts[k], pts[(k + 1) % len(pts)])
+
+    def arrays(self):
+        v = np.array(self._v, dtype=np.float32).reshape(-1, 2)
+        return SceneArrays(v, self._cpos, self._ctype, self._cpath, self._cw, self._rule,
+                           self._color)
+
+
+KIND_CUBIC_CHAINS, KIND_MIXED, KIND_GLYPHS = 1, 2, 3
+
+
+def generate(kind, width, height, n_paths, seed, min_segs=0, max_segs=0):
+    """Seeded synthetic scene from the library generator, copied into numpy arrays."""
+    L = _abi.lib()
+    obj = ctypes.c_void_p()
+    rc = L.sl_scene_generate(kind, width, height, n_paths, ctypes.c_uint64(seed), min_segs,
+                             max_segs, ctypes.byref(obj))
+    if rc:
+        raise _abi.ScanlineError(rc, "sl_scene_generate failed")
+    try:
+        s = _abi.SlScene()
+        L.sl_scene_view(obj, ctypes.byref(s))
+
+        def arr(ptr, n, ct, dt):
+            if n == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), (n,)).astype(dt, copy=True)
+
+        V, C, P = s.n_vertices, s.n_curves, s.n_paths
+        out = SceneArrays(arr(s.vertex, 2 * V, ctypes.c_float, np.float32),
+                          arr(s.curve_vertex_pos, C, ctypes.c_uint32, np.uint32),
+                          arr(s.curve_type, C, ctypes.c_uint8, np.uint8),
+                          arr(s.curve_path_id, C, ctypes.c_uint32, np.uint32),
+                          arr(s.curve_arc_w, C, ctypes.c_float, np.float32),
+                          arr(s.fill_rule, P, ctypes.c_uint8, np.uint8),
+                          arr(s.fill_color, P, ctypes.c_uint32, np.uint32))
+    finally:
+        L.sl_scene_free(obj)
+    return out
+
+
+SEED_BASE = 0x5CA1AB1E
+
+
+def config_scene(config_id, scale=1.0):
+    """The benchmark workloads' scenes (kind, canvas, paths, spp) for configs 1-3 and 5.
+
+    ``scale`` < 1 shrinks path counts for quick parity runs at the same density rules.
+    """
+    seed = SEED_BASE ^ config_id
+    if config_id == 1:
+        return generate(KIND_CUBIC_CHAINS, 512, 512, max(1, int(200 * scale)), seed), 512, 512, 8
+    if config_id == 2:
+        return generate(KIND_MIXED, 2048, 2048, max(1, int(20000 * scale)), seed), 2048, 2048, 32
+    if config_id == 3:
+        return generate(KIND_GLYPHS, 4096, 4096, max(1, int(100000 * scale)), seed), 4096, 4096, 16
+    if config_id == 5:
+        return generate(KIND_CUBIC_CHAINS, 256, 256, max(1, int(64 * scale)), seed, 6, 6), 256, 256, 16
+    raise ValueError("no single-scene workload for config %d" % config_id)

@@ -1,0 +1,109 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded by tests/, the smoke check and the benchmark's CPU-baseline leg -- as the checker
+or the timed CPU baseline, never as the renderer.  ``build()`` compiles it with the
+Makefile beside this file.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_SERIAL = os.path.join(HERE, "build", "liboracle.so")
+LIB_OMP = os.path.join(HERE, "build", "liboracle_omp.so")
+
+_DT = {
+    "tvertex": (np.float32, 2), "path_flags": (np.uint8, 8), "cuts": (np.float32, 4),
+    "ncuts": (np.uint8, 1), "frag_count": (np.uint32, 1), "ts": (np.int32, 2),
+    "frag_yx": (np.int32, 1), "frag_pid": (np.int32, 1), "frag_wn": (np.int8, 1),
+    "frag_mask": (np.uint32, 4), "sorted_yx": (np.int32, 1), "sorted_idx": (np.int32, 1),
+    "wn_scan": (np.int8, 1), "merged_mask": (np.uint32, 4), "draw_list": (np.int32, 4),
+    "image": (np.uint8, 4), "counts": (np.int64, 1), "pixel_count": (np.int64, 1),
+}
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+_libs = {}
+
+
+def lib(omp=False):
+    path = LIB_OMP if omp else LIB_SERIAL
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    vp, u32, i = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    L.oracle_run.restype = vp
+    L.oracle_run.argtypes = [u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, i, i, i, i, i, vp, i]
+    L.oracle_get.restype = i
+    L.oracle_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t)]
+    L.oracle_render_seconds.restype = ctypes.c_double
+    L.oracle_render_seconds.argtypes = [vp]
+    L.oracle_free.argtypes = [vp]
+    L.oracle_threads.restype = i
+    L.oracle_sample_positions.argtypes = [i, vp]
+    L.oracle_pixel_table.argtypes = [i, vp]
+    L.oracle_fetch_pixel.argtypes = [i, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]
+    L.oracle_fetch_pixel_batch.argtypes = [i, ctypes.c_int64, vp, vp]
+    L.oracle_vertical_mask.argtypes = [i, ctypes.c_float, vp]
+    L.oracle_vertical_mask2.argtypes = [i, ctypes.c_float, ctypes.c_float, vp]
+    L.oracle_solve_quad.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.oracle_curve_eval.restype = ctypes.c_float
+    L.oracle_curve_eval.argtypes = [i, ctypes.c_float, vp, ctypes.c_float]
+    L.oracle_color_tables.argtypes = [i, vp, vp]
+    _libs[path] = L
+    return L
+
+
+class OracleResult:
+    def __init__(self, L, ptr):
+        self._L, self._p = L, ptr
+
+    def get(self, name):
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        if self._L.oracle_get(self._p, name.encode(), ctypes.byref(p), ctypes.byref(n)):
+            raise KeyError(name)
+        dt, width = _DT[name]
+        cnt = n.value // np.dtype(dt).itemsize
+        if cnt == 0:
+            return np.zeros((0, width) if width > 1 else 0, dtype=dt)
+        a = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dt))), (cnt,)).copy()
+        return a.reshape(-1, width) if width > 1 else a
+
+    @property
+    def seconds(self):
+        return self._L.oracle_render_seconds(self._p)
+
+    def counts(self):
+        c = self.get("counts")
+        keys = ("path", "curve", "vertex", "curve_fragment", "merged_fragment", "span")
+        return {k: int(v) for k, v in zip(keys, c)}
+
+    def __del__(self):
+        try:
+            self._L.oracle_free(self._p)
+        except Exception:
+            pass
+
+
+def run(scene, width, height, samples, srgb=True, ms=False, transform=None, keep=True, omp=False):
+    """Render `scene` (scanline.scene.SceneArrays) on the CPU oracle."""
+    L = lib(omp)
+    P, C, V, vtx, cvpos, ctype, cpath, cw, rule, color = scene.oracle_args()
+    m = None
+    if transform is not None:
+        m = np.ascontiguousarray(np.asarray(transform, dtype=np.float32).reshape(16))
+    ptr = L.oracle_run(P, C, V, vtx.ctypes.data, cvpos.ctypes.data, ctype.ctypes.data, cpath.ctypes.data,
+                       cw.ctypes.data, rule.ctypes.data, color.ctypes.data, int(width), int(height),
+                       int(samples), int(bool(srgb)), int(bool(ms)),
+                       m.ctypes.data if m is not None else None, 1 if keep else 0)
+    if not ptr:
+        raise ValueError("oracle_run rejected the arguments")
+    return OracleResult(L, ptr)

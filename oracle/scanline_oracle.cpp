@@ -1276,6 +1276,15 @@ void oracle_fetch_pixel(int n, float p0x, float p0y, float p1x, float p1y, uint3
     fetch_pixel(T, p0x, p0y, p1x, p1y, out);
 }
 
+void oracle_fetch_pixel_batch(int n, int64_t count, const float *pts, uint32_t *out) {
+    Tables T;
+    T.n = n;
+    T.pixel.assign((size_t)65536 * words_for(n), 0);
+    make_pixel_table(n, T.pixel.data());
+    for (int64_t i = 0; i < count; ++i)
+        fetch_pixel(T, pts[4 * i], pts[4 * i + 1], pts[4 * i + 2], pts[4 * i + 3], out + 4 * i);
+}
+
 void oracle_vertical_mask(int n, float y, uint32_t out[4]) { vertical_mask(n, y, out); }
 void oracle_vertical_mask2(int n, float y0, float y1, uint32_t out[4]) { vertical_mask2(n, y0, y1, out); }
 void oracle_solve_quad(float a, float b, float c, float *r0, float *r1) { solve_quad(a, b, c, r0, r1); }

@@ -46,6 +46,7 @@ int oracle_sample_positions(int n, float *xy);
 /* 256x256 line->mask table: words_per_texel = 1 (8 spp), 2 (16 spp), 4 (32 spp). */
 int oracle_pixel_table(int n, uint32_t *out);
 void oracle_fetch_pixel(int n, float p0x, float p0y, float p1x, float p1y, uint32_t out[4]);
+void oracle_fetch_pixel_batch(int n, int64_t count, const float *pts, uint32_t *out);
 void oracle_vertical_mask(int n, float y, uint32_t out[4]);
 void oracle_vertical_mask2(int n, float y0, float y1, uint32_t out[4]);
 void oracle_solve_quad(float a, float b, float c, float *r0, float *r1);
