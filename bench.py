@@ -1,0 +1,179 @@
+"""Benchmark of the comb-scanline renderer on one MI355X.
+
+Workload (the plain renderer's headline configuration): 2048x2048 canvas, 20,000 seeded
+paths (40% lines / 30% quadratics / 30% cubics, 3-64 segments, log-uniform 4-1024 px
+bboxes, 10% star polygons, alternating nonzero / even-odd, alpha in [0.2, 1]), 32 samples
+per pixel, coverage-alpha compositing into an sRGB8 image.  One step = one full render
+(transform through the RGBA8 image) of the scene already resident in HBM.
+
+Prints one JSON line: Mfragments/s (raw curve fragments F per second) with Mpixels/s
+beside it, the dominant kernel's roofline entry (algorithmic bytes / its HIP-event time
+against the 8 TB/s HBM3E peak) and the CPU baseline (the OpenMP build of the oracle on a
+bounded sample of the same scene).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-scanline-path-rendering_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(stage, cnt, spp, W, H):
+    """Algorithmic HBM bytes of one stage's launch (each datum read or written once)."""
+    m = spp // 2  # mask bytes per 2x2 cell
+    F, M, S = cnt["curve_fragment"], cnt["merged_fragment"], cnt["span"]
+    C, V, P = cnt["curve"], cnt["vertex"], cnt["path"]
+    items = M + S
+    if stage == "intersect":      # curve (type, vpos, path, cuts, count, offset) + vertices, ts out
+        return 30 * C + 8 * V + 8 * F
+    if stage == "fragments":      # ts in; per fragment record out (meta 4 + mask m)
+        return 8 * F + (4 + m) * F
+    if stage == "resolve":        # records in; items (4 B) + resolved masks (m) out
+        return (4 + m) * F + 4 * items + m * M
+    if stage == "composite":      # items + resolved masks in; RGBA8 image out
+        return 4 * items + m * M + 4 * W * H
+    if stage == "transform_cull":
+        return 8 * V + 4 * V + 8 * V + 4 * P
+    if stage == "monotone_count":
+        return 30 * C + 8 * V + 21 * C
+    return 0
+
+
+def b_alg_total(cnt, spp, W, H):
+    m = spp // 2
+    F, M, S = cnt["curve_fragment"], cnt["merged_fragment"], cnt["span"]
+    return (8 * cnt["vertex"] + 14 * cnt["curve"] + 5 * cnt["path"] + F * 2 * (8 + m + 9)
+            + M * 2 * m + S * 2 * 8 + W * H * 4)
+
+
+def cpu_baseline(scene_full, W, H, spp, sample_paths, seed):
+    """OpenMP oracle on the first `sample_paths` paths of the same scene generator."""
+    import pyoracle
+    import scanline
+
+    sub = scanline.generate(scanline.scene.KIND_MIXED, W, H, sample_paths, seed)
+    L = pyoracle.lib(omp=True)
+    threads = L.oracle_threads()
+    times, frags = [], 0
+    for _ in range(3):
+        r = pyoracle.run(sub, W, H, spp, srgb=True, ms=False, keep=False, omp=True)
+        times.append(r.seconds)
+        frags = r.counts()["curve_fragment"]
+        del r
+    t = sorted(times)[len(times) // 2]
+    return {"value": round(frags / t / 1e6, 3), "unit": "Mfragments/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d of %d paths of the same seeded scene at %dx%d, %d spp, coverage output; "
+                      "median of 3 renders (%.2f s each, F=%d)" % (sample_paths, scene_full.n_paths, W, H,
+                                                                  spp, t, frags)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--paths", type=int, default=20000)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--spp", type=int, default=32)
+    ap.add_argument("--ms", action="store_true", help="multisample (per-sample) compositing")
+    ap.add_argument("--cpu-sample-paths", type=int, default=2000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this renderer runs on one GPU; --gpus %d is not supported "
+              "(the multi-GPU band sharding is a design only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import scanline
+
+    W = H = args.size
+    seed = scanline.scene.SEED_BASE ^ 2
+    scene = scanline.generate(scanline.scene.KIND_MIXED, W, H, args.paths, seed)
+    r = scanline.VGRasterizer(0)
+    r.setSamples(args.spp)
+    r.setMultisampleOutput(args.ms)
+    r.enableSRGBCorrection(True)
+    r.resizeOutputBuffer(W, H)
+    r.addVg(scene)
+    r.enableStepTiming(True)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    for _ in range(args.warmup):
+        r.rasterize(stream)
+    torch.cuda.synchronize()
+    stage_sum = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.rasterize(stream)
+        for k, v in r.stepTiming().items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1e3
+    cnt = r.elementNumber()
+    stages = {k: v / args.steps for k, v in stage_sum.items()}
+
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        a = torch.empty(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+
+    F = cnt["curve_fragment"]
+    value = F / (ms_step * 1e-3) / 1e6
+    dom = max((k for k in stages if stage_bytes(k, cnt, args.spp, W, H) > 0), key=lambda k: stages[k])
+    dom_bytes = stage_bytes(dom, cnt, args.spp, W, H)
+    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
+    b_alg = b_alg_total(cnt, args.spp, W, H)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, W, H, args.spp, args.cpu_sample_paths, seed)
+
+    out = {
+        "metric": "Mfragments/s + Mpixels/s at 2048^2 32spp (GB/s vs HBM peak); bwd/fwd time",
+        "value": round(value, 3),
+        "unit": "Mfragments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, splitmix64 seed 0x5CA1AB1E^2)",
+        "config": {"workload": "plain renderer, 2048x2048 canvas, 20k mixed paths, 32 spp",
+                   "width": W, "height": H, "paths": args.paths, "spp": args.spp,
+                   "output": "multisample" if args.ms else "coverage", "srgb": True},
+        "mpixels_per_s": round(W * H / (ms_step * 1e-3) / 1e6, 3),
+        "counts": cnt,
+        "b_alg_bytes": int(b_alg),
+        "whole_frame_gbs": round(b_alg / (ms_step * 1e-3) / 1e9, 2),
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out))
+    r.close()
+
+
+if __name__ == "__main__":
+    main()
