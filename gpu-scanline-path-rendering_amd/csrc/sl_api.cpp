@@ -59,6 +59,7 @@ struct sl_handle {
     DevBuf ptable[3];
     DevBuf ctab[2];  // [srgb] : dec_c, thr_c, dec_a, thr_a, lut_c, lut_a packed
     // per-frame arena
+    DevBuf b_pcnt, b_nchain, b_choff, b_chains, b_chain_t;
     DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
         b_boff, b_bwsum, b_bcarry, b_bcursor, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_bicnt, b_bmcnt,
         b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
@@ -216,16 +217,21 @@ int render(sl_handle *h, hipStream_t s) {
     f.pitch = h->W * 4;
 
     size_t scan_need = sl::scan_tmp_elems_needed((size_t)std::max(C, P) + 1);
+    int rc2 = SL_OK;
+    (void)rc2;
     if ((rc = ensure(h, h->b_tvtx, (size_t)V * 8)) || (rc = ensure(h, h->b_pflags, (size_t)P * 4)) ||
         (rc = ensure(h, h->b_prow, (size_t)P * 8)) || (rc = ensure(h, h->b_ccuts, (size_t)C * 16)) ||
         (rc = ensure(h, h->b_cncut, (size_t)C)) || (rc = ensure(h, h->b_cfrag, ((size_t)C + 1) * 4)) ||
         (rc = ensure(h, h->b_coff, ((size_t)C + 1) * 4)) || (rc = ensure(h, h->b_pnb, ((size_t)P + 1) * 4)) ||
+        (rc = ensure(h, h->b_pcnt, (size_t)C * 20)) || (rc = ensure(h, h->b_nchain, ((size_t)C + 1) * 4)) ||
+        (rc = ensure(h, h->b_choff, ((size_t)C + 1) * 4)) ||
         (rc = ensure(h, h->b_pbase, ((size_t)P + 1) * 4)) || (rc = ensure(h, h->b_small, 64)) ||
         (rc = ensure(h, h->b_image, (size_t)h->H * f.pitch)) || (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
         return rc;
     f.tvtx = P_<float2>(h->b_tvtx); f.pflags = P_<uint32_t>(h->b_pflags); f.prow = P_<int2>(h->b_prow);
     f.ccuts = P_<float4>(h->b_ccuts); f.cncut = P_<uint8_t>(h->b_cncut); f.cfrag = P_<uint32_t>(h->b_cfrag);
     f.coff = P_<uint32_t>(h->b_coff); f.pnb = P_<uint32_t>(h->b_pnb); f.pbase = P_<uint32_t>(h->b_pbase);
+    f.pcnt = P_<uint32_t>(h->b_pcnt); f.nchain = P_<uint32_t>(h->b_nchain); f.choff = P_<uint32_t>(h->b_choff);
     f.image = P_<uint8_t>(h->b_image);
     f.big_count = P_<uint32_t>(h->b_small);
     f.err_flag = P_<uint32_t>(h->b_small) + 1;
@@ -242,11 +248,14 @@ int render(sl_handle *h, hipStream_t s) {
     mark(1);
     sl::launch_monotone(sc, f, s);
     sl::scan_exclusive_u32(f.cfrag, f.coff, C, f.scan_tmp, s);
+    sl::scan_exclusive_u32(f.nchain, f.choff, C, f.scan_tmp, s);
     if ((rc = check_launch(h, "monotone"))) return rc;
-    rc = SL_OK;
-    const uint32_t F = read_u32(h, f.coff + C, s, rc);
-    if (rc) return rc;
+    uint32_t F = 0, NCH = 0;
+    HIPCHK(h, hipMemcpyAsync(&F, f.coff + C, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipMemcpyAsync(&NCH, f.choff + C, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
     f.F = F;
+    f.NCH = NCH;
     h->counts.path = P; h->counts.curve = C; h->counts.vertex = V;
     h->counts.curve_fragment = F;
     h->counts.merged_fragment = 0;
@@ -261,8 +270,12 @@ int render(sl_handle *h, hipStream_t s) {
         HIPCHK(h, hipStreamSynchronize(s));
         return SL_OK;
     }
-    if ((rc = ensure(h, h->b_ts, (size_t)F * 8))) return rc;
+    if ((rc = ensure(h, h->b_ts, (size_t)F * 8)) || (rc = ensure(h, h->b_chains, (size_t)NCH * 4)) ||
+        (rc = ensure(h, h->b_chain_t, (size_t)F * 4)))
+        return rc;
     f.ts = P_<int2>(h->b_ts);
+    f.chains = P_<uint32_t>(h->b_chains);
+    f.chain_t = P_<float>(h->b_chain_t);
     mark(2);
     sl::launch_intersect(sc, f, s);
     if ((rc = check_launch(h, "intersect"))) return rc;
@@ -439,7 +452,7 @@ int sl_destroy(sl_handle *h) {
     if (!h) return SL_OK;
     (void)hipSetDevice(h->device);
     (void)hipDeviceSynchronize();
-    DevBuf *bufs[] = {&h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->frule, &h->finfo,
+    DevBuf *bufs[] = {&h->b_pcnt, &h->b_nchain, &h->b_choff, &h->b_chains, &h->b_chain_t, &h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->frule, &h->finfo,
                       &h->ptable[0], &h->ptable[1], &h->ptable[2], &h->ctab[0], &h->ctab[1], &h->b_tvtx, &h->b_pflags,
                       &h->b_prow, &h->b_ccuts, &h->b_cncut, &h->b_cfrag, &h->b_coff, &h->b_ts, &h->b_pnb, &h->b_pbase,
                       &h->b_binfo, &h->b_bcount, &h->b_boff, &h->b_bwsum, &h->b_bcarry, &h->b_bcursor, &h->b_rec_meta,

@@ -239,9 +239,12 @@ __device__ __forceinline__ float pick5(int i, float a, float b, float c, float d
     return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : (i == 3 ? d : e)));
 }
 
-// k_make_intersection_0 (ras_cut.h:1254-1482): cuts + fragment count per curve
+// k_make_intersection_0 (ras_cut.h:1254-1482): monotone cuts, then per monotone piece the
+// numbers of even x / y cut lines it crosses (d_udpate_cut_range) and the curve's fragment
+// count.  Also the number of solve chains (two per piece) of the intersection pass.
 __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__restrict__ tv, const uint32_t *__restrict__ pflags,
-                                                   int W, int H, float4 *ccuts, uint8_t *cncut, uint32_t *cfrag) {
+                                                   int W, int H, float4 *ccuts, uint8_t *cncut, uint32_t *cfrag,
+                                                   uint32_t *pcnt, uint32_t *nchain) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= sc.C) return;
     Curve cu = load_curve(sc, tv, c);
@@ -279,138 +282,190 @@ __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__r
     ccuts[c] = make_float4(q0, q1, q2, q3);
     cncut[c] = (uint8_t)n;
     uint32_t cnt = 0;
+    uint32_t pc[5] = {0u, 0u, 0u, 0u, 0u};
+    const int npieces = vis ? n + 1 : 0;
     if (vis) {
-        const int npieces = n + 1;
         float v0x = cu.x0, v0y = cu.y0;
         for (int ms = 0; ms < npieces; ++ms) {
             float t1 = (ms == n) ? 1.f : pick5(ms, q0, q1, q2, q3, 1.f);
             float v1x = ex(cu, t1), v1y = ey(cu, t1);
             CutRange r = cut_range(v0x, v0y, v1x, v1y, W, H);
             cnt += 1u + (uint32_t)r.nx + (uint32_t)r.ny;
+            pc[ms] = (uint32_t)r.nx | ((uint32_t)r.ny << 16);
             v0x = v1x; v0y = v1y;
         }
     }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pcnt[5 * (size_t)c + k] = pc[k];
     cfrag[c] = cnt;
+    nchain[c] = 2u * (uint32_t)npieces;
 }
 
-// k_make_intersection_1 (ras_cut.h:837-1241, 1484-1562): one thread per curve writes
-// the curve's {curve id, t | flags} entries.  While emitting, it evaluates each
-// fragment's chord midpoint row (as the fragment kernel will) to record the per-path
-// range of valid cell rows.
-__global__ void __launch_bounds__(256) k_intersect(SceneDev sc, const float2 *__restrict__ tv,
-                                                    const float4 *__restrict__ ccuts, const uint8_t *__restrict__ cncut,
-                                                    const uint32_t *__restrict__ coff, int W, int H, int2 *ts, int2 *prow) {
+// one descriptor per (curve, piece, side) solve chain: curve << 4 | piece << 1 | side
+__global__ void k_chain_list(uint32_t C, const uint32_t *__restrict__ nchain, const uint32_t *__restrict__ choff,
+                             uint32_t *chains) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= sc.C) return;
-    const uint32_t base = coff[c];
-    const int64_t cap = (int64_t)(coff[c + 1] - base);
-    if (cap == 0) return;
+    if (c >= C) return;
+    const uint32_t n = nchain[c], o = choff[c];
+    for (uint32_t k = 0; k < n; ++k) chains[o + k] = (c << 4) | k;
+}
+
+// t | flags of a monotone piece's end: low bits 2 when the end lies on an integral x,
+// else 3 (k_make_intersection_impl_1, ras_cut.h:858-865)
+__device__ __forceinline__ float piece_end_flags(float t_raw, float vx) {
+    float t = bf(fb(t_raw) & 0xFFFFFFFCu);
+    if (floorf(vx) == vx) return bf((fb(t) & 0xFFFFFFFCu) | 2u);
+    return bf(fb(t) | 3u);
+}
+
+// One cut solve on one coordinate of the curve (ras_cut.h:1004-1106): line and
+// quadratic / rational in closed form, cubic by 24-step bisection from t_min.
+__device__ __forceinline__ float solve_cut(int type, float w, float a0, float a1, float a2, float a3, float cc,
+                                           float t_min, float t1) {
+    if (type == CT_LINEAR) return tmin(tmax((cc - a0) * safe_rcp(a1 - a0), t_min), t1);
+    if (type == CT_QUAD) return solve_quad_bezier(a0, a1, a2, cc, t_min, t1);
+    if (type == CT_RATIONAL) return solve_quad_bezier(a0 - cc, (a1 - cc) * w, a2 - cc, 0.f, t_min, t1);
+    float lo = t_min, hi = t1;
+    float vlo = curve_eval(CT_CUBIC, 0.f, a0, a1, a2, a3, lo);
+    float t_solve = lo;
+    if (vlo != cc) {
+        float last_v = 0.f;
+        for (int it = 0; it < 24; ++it) {
+            float tm = (lo + hi) * .5f;
+            float vm = curve_eval(CT_CUBIC, 0.f, a0, a1, a2, a3, tm);
+            t_solve = tm;
+            last_v = vm;
+            if ((int32_t)(fb(vm - cc) ^ fb(vlo - cc)) >= 0) { lo = tm; vlo = vm; }
+            else { hi = tm; }
+        }
+        if (fabsf(last_v - cc) > 1.f) t_solve = t_min;
+    }
+    return t_solve;
+}
+
+// Intersection pass 1: one thread per (curve, piece, side) chain.  In the reference's
+// merge walk (ras_cut.h:910-1232) the j-th cut on one side is solved from the t the same
+// side produced last (or the piece start), so each side's chain is independent of the
+// other side's: solve it sequentially, storing t | side for the merge pass.
+__global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
+                                                 const uint8_t *__restrict__ cncut, const uint32_t *__restrict__ coff,
+                                                 const uint32_t *__restrict__ pcnt, const uint32_t *__restrict__ chains,
+                                                 const uint32_t *nch_ptr, int W, int H, float *chain_t) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *nch_ptr) return;
+    const uint32_t d = chains[i];
+    const uint32_t c = d >> 4;
+    const int k = (int)((d >> 1) & 7u), side = (int)(d & 1u);
+    const uint32_t *pc = pcnt + 5 * (size_t)c;
+    uint32_t off = coff[c];
+    for (int kk = 0; kk < k; ++kk) off += 1u + (pc[kk] & 0xFFFFu) + (pc[kk] >> 16);
+    const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
+    const uint32_t cnt = side ? ny : nx;
+    if (cnt == 0) return;
     Curve cu = load_curve(sc, tv, c);
     const float4 q = ccuts[c];
     const int n = cncut[c];
+    float v0x = cu.x0, v0y = cu.y0, t0m = 0.f;
+    if (k > 0) {
+        const float tr = pick5(k - 1, q.x, q.y, q.z, q.w, 1.f);
+        v0x = ex(cu, tr);
+        v0y = ey(cu, tr);
+        t0m = piece_end_flags(tr, v0x);
+    }
+    const float t1r = (k == n) ? 1.f : pick5(k, q.x, q.y, q.z, q.w, 1.f);
+    const float v1x = ex(cu, t1r), v1y = ey(cu, t1r);
+    const float t1m = piece_end_flags(t1r, v1x);
+    const CutRange r = cut_range(v0x, v0y, v1x, v1y, W, H);
+    float cc = side ? (float)(r.dy < 0.f ? r.ye : r.yb) : (float)(r.dx < 0.f ? r.xe : r.xb);
+    const float dd = side ? r.dy : r.dx;
+    const float a0 = side ? cu.y0 : cu.x0, a1 = side ? cu.y1 : cu.x1;
+    const float a2 = side ? cu.y2 : cu.x2, a3 = side ? cu.y3 : cu.x3;
+    float *out = chain_t + off + (side ? nx : 0u);
+    float t_min = t0m;
+    for (uint32_t j = 0; j < cnt; ++j) {
+        const float t = solve_cut(cu.type, cu.w, a0, a1, a2, a3, cc, t_min, t1m);
+        const float tm = bf((fb(t) & 0xFFFFFFFCu) | (uint32_t)side);
+        out[j] = tm;
+        t_min = tm;
+        cc += dd;
+    }
+}
+
+// Intersection pass 2: one thread per curve replays the reference's slot walk
+// (ras_cut.h:910-1000) over the solved chains: pick the x slot when x <= y, emit its t,
+// refill it from the chain or with the exhausted marker 2 | side; equal consecutive t's
+// OR their flags into the previous entry.  It also records the range of cell rows of the
+// curve's fragments (chord midpoints, .cu:247-262) for the (path, row) bucketing.
+__global__ void __launch_bounds__(256) k_merge(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
+                                                const uint8_t *__restrict__ cncut, const uint32_t *__restrict__ coff,
+                                                const uint32_t *__restrict__ pcnt, const float *__restrict__ chain_t, int H,
+                                                int2 *ts, int2 *prow) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= sc.C) return;
+    const uint32_t base = coff[c];
+    if (coff[c + 1] == base) return;
+    Curve cu = load_curve(sc, tv, c);
+    const float4 q = ccuts[c];
+    const int n = cncut[c];
+    const uint32_t *pc = pcnt + 5 * (size_t)c;
     int2 *out = ts + base;
+    const float *tt = chain_t + base;
     int rmin = INT_MAX, rmax = INT_MIN;
     bool have_prev = false;
     float prev_t = 0.f, prev_y = 0.f;
-    auto emit_row = [&](float t) {
-        // fragment (prev_t, t) exists iff prev_t < t; its cell row comes from the
-        // midpoint of its chord (k_gen_fragment_and_stencil_mask, .cu:247-262)
-        float y = ey(cu, t);
+    auto track = [&](float t) {
+        const float y = ey(cu, t);
         if (have_prev && prev_t < t) {
-            int32_t ry = floor_even((prev_y + y) * 0.5f);
+            const int32_t ry = floor_even((prev_y + y) * 0.5f);
             if ((uint32_t)ry < (uint32_t)H) {
-                int r = ry >> 1;
-                rmin = tmin(rmin, r);
-                rmax = tmax(rmax, r);
+                rmin = tmin(rmin, ry >> 1);
+                rmax = tmax(rmax, ry >> 1);
             }
         }
         have_prev = true;
         prev_t = t;
         prev_y = y;
     };
-
-    float t0_ms = 0.f;
-    float v0x = cu.x0, v0y = cu.y0;
-    int64_t p = 0;
-    for (int ms = 0; ms <= n; ++ms) {
-        float t1_ms = (ms == n) ? 1.f : pick5(ms, q.x, q.y, q.z, q.w, 1.f);
-        float v1x = ex(cu, t1_ms), v1y = ey(cu, t1_ms);
-        t1_ms = bf(fb(t1_ms) & 0xFFFFFFFCu);
-        if (floorf(v1x) == v1x) t1_ms = bf((fb(t1_ms) & 0xFFFFFFFCu) | 2u);
-        else t1_ms = bf(fb(t1_ms) | 3u);
-        CutRange r = cut_range(v0x, v0y, v1x, v1y, W, H);
-        int nx = r.nx, ny = r.ny;
-        const int n_loop = nx + ny + 1;
-        float x = (float)(r.dx < 0.f ? r.xe : r.xb);
-        float y = (float)(r.dy < 0.f ? r.ye : r.yb);
-        float slot_x = t0_ms, slot_y = t0_ms;
+    const float two = 2.f, two1 = bf(fb(2.f) | 1u);
+    uint32_t p = 0;
+    float t0m = 0.f;
+    for (int k = 0; k <= n; ++k) {
+        const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
+        const float t1r = (k == n) ? 1.f : pick5(k, q.x, q.y, q.z, q.w, 1.f);
+        const float t1m = piece_end_flags(t1r, ex(cu, t1r));
+        const float *X = tt + p;
+        const float *Y = X + nx;
+        float xs, ys;
+        uint32_t xi = 0, yi = 0, rx, ry;
+        if (nx == 0) { xs = two; ys = t0m; rx = 0; ry = ny; }
+        else if (ny == 0) { xs = t0m; ys = two1; rx = nx; ry = 0; }
+        else { xs = X[0]; xi = 1; rx = nx - 1; ys = t0m; ry = ny; }
         uint32_t last = fb(-1.f);
-        for (int i = -1; i < n_loop; ++i) {
-            float t_solve = 0.f, cc = 0.f;
-            int side = 0;
-            float t_min;
-            if (i == -1) {
-                t_min = slot_x;
-                if (nx == 0) { side = 0; t_solve = 2.f; }
-                else if (ny == 0) { side = 1; t_solve = 2.f; }
-                else { side = 0; nx--; cc = x; x += r.dx; }
-            } else if (slot_x <= slot_y) {
-                t_min = slot_x;
-                side = 0;
-                if (nx > 0) { nx--; cc = x; x += r.dx; } else { t_solve = 2.f; }
+        const uint32_t n_loop = nx + ny + 1;
+        for (uint32_t it = 0; it < n_loop; ++it) {
+            float v;
+            if (xs <= ys) {
+                v = xs;
+                if (rx > 0) { xs = X[xi++]; rx--; } else { xs = two; }
             } else {
-                t_min = slot_y;
-                side = 1;
-                if (ny > 0) { ny--; cc = y; y += r.dy; } else { t_solve = 2.f; }
+                v = ys;
+                if (ry > 0) { ys = Y[yi++]; ry--; } else { ys = two1; }
             }
-            if (i >= 0 && p < cap) {
-                uint32_t o = fb(t_min);
-                if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
-                    o |= last;
-                    out[p - 1].y = (int)o;
-                }
-                out[p] = make_int2((int)c, (int)o);
-                last = o;
-                p++;
-                float tm = bf(o & 0xFFFFFFFCu);
-                if (tm < 0.f) tm = 0.f;
-                emit_row(tm);
+            uint32_t o = fb(v);
+            if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
+                o |= last;
+                out[p - 1].y = (int)o;
             }
-            if (t_solve < 2.f) {
-                const float a0 = side ? cu.y0 : cu.x0, a1 = side ? cu.y1 : cu.x1;
-                const float a2 = side ? cu.y2 : cu.x2, a3 = side ? cu.y3 : cu.x3;
-                if (cu.type == CT_LINEAR) {
-                    t_solve = tmin(tmax((cc - a0) * safe_rcp(a1 - a0), t_min), t1_ms);
-                } else if (cu.type == CT_QUAD) {
-                    t_solve = solve_quad_bezier(a0, a1, a2, cc, t_min, t1_ms);
-                } else if (cu.type == CT_RATIONAL) {
-                    t_solve = solve_quad_bezier(a0 - cc, (a1 - cc) * cu.w, a2 - cc, 0.f, t_min, t1_ms);
-                } else {
-                    float lo = t_min, hi = t1_ms;
-                    float vlo = curve_eval(CT_CUBIC, 0.f, a0, a1, a2, a3, lo);
-                    t_solve = lo;
-                    if (vlo != cc) {
-                        const float raw = lo;
-                        float last_v = 0.f;
-                        for (int it = 0; it < 24; ++it) {
-                            float tm = (lo + hi) * .5f;
-                            float vm = curve_eval(CT_CUBIC, 0.f, a0, a1, a2, a3, tm);
-                            t_solve = tm;
-                            last_v = vm;
-                            if ((int32_t)(fb(vm - cc) ^ fb(vlo - cc)) >= 0) { lo = tm; vlo = vm; }
-                            else { hi = tm; }
-                        }
-                        if (fabsf(last_v - cc) > 1.f) t_solve = raw;
-                    }
-                }
-            }
-            const float nv = bf((fb(t_solve) & 0xFFFFFFFCu) | (uint32_t)side);
-            if (side) slot_y = nv; else slot_x = nv;
+            out[p] = make_int2((int)c, (int)o);
+            last = o;
+            p++;
+            float tm = bf(o & 0xFFFFFFFCu);
+            if (tm < 0.f) tm = 0.f;
+            track(tm);
         }
-        t0_ms = t1_ms;
-        v0x = v1x; v0y = v1y;
+        t0m = t1m;
     }
-    emit_row(1.f);  // the curve's last fragment ends at t = 1 (.cu:148)
+    track(1.f);  // the curve's last fragment ends at t = 1 (.cu:148)
     if (rmin <= rmax) {
         const uint32_t pth = sc.cpath[c];
         atomicMin(&prow[pth].x, rmin);
@@ -458,13 +513,20 @@ void launch_transform(const SceneDev &sc, Frame &f, hipStream_t s) {
 void launch_monotone(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.C) return;
     hipLaunchKernelGGL(k_monotone, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                       (const uint32_t *)f.pflags, f.W, f.H, f.ccuts, f.cncut, f.cfrag);
+                       (const uint32_t *)f.pflags, f.W, f.H, f.ccuts, f.cncut, f.cfrag, f.pcnt, f.nchain);
 }
 
 void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.C) return;
-    hipLaunchKernelGGL(k_intersect, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                       (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff, f.W, f.H, f.ts, f.prow);
+    hipLaunchKernelGGL(k_chain_list, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc.C, (const uint32_t *)f.nchain,
+                       (const uint32_t *)f.choff, f.chains);
+    const uint32_t *nch = f.choff + sc.C;
+    hipLaunchKernelGGL(k_chains, dim3(blocks_for(f.NCH ? f.NCH : 1, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
+                       (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
+                       (const uint32_t *)f.pcnt, (const uint32_t *)f.chains, nch, f.W, f.H, f.chain_t);
+    hipLaunchKernelGGL(k_merge, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
+                       (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
+                       (const uint32_t *)f.pcnt, (const float *)f.chain_t, f.H, f.ts, f.prow);
 }
 
 }  // namespace sl

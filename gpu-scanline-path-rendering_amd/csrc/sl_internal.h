@@ -45,6 +45,12 @@ struct Frame {
     uint8_t *cncut;          // [C]
     uint32_t *cfrag;         // [C+1]  -> exclusive offsets after scan
     uint32_t *coff;          // [C+1]
+    uint32_t *pcnt;          // [5C] per monotone piece: x cuts | y cuts << 16
+    uint32_t *nchain;        // [C+1] solve chains per curve (2 per piece)
+    uint32_t *choff;         // [C+1] exclusive scan of nchain
+    uint32_t *chains;        // [NCH] chain descriptors
+    int64_t NCH;
+    float *chain_t;          // [F] solved t | side per chain
     // fragments
     int64_t F;
     int2 *ts;                // [F]

@@ -15,6 +15,8 @@
 #include "sl_device.h"
 #include "sl_internal.h"
 
+#include <algorithm>
+
 namespace sl {
 using namespace dev;
 
@@ -125,120 +127,189 @@ struct ResolveArgs {
     uint32_t *err;
 };
 
-// One wavefront per bucket of at most 64 fragments (the common case): rank-sort by cell x
-// in registers, then per group (cell) leaders accumulate the members' masks.
-template <int SPP>
-__global__ void __launch_bounds__(256) k_resolve_small(ResolveArgs A) {
+// Wave-wide inclusive scans / reductions with DPP row shifts and row broadcasts.
+template <class Op>
+__device__ __forceinline__ int dpp_scan(int x, int ident, Op op) {
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+struct OpAdd { __device__ int operator()(int a, int b) const { return a + b; } };
+struct OpMin { __device__ int operator()(int a, int b) const { return a < b ? a : b; } };
+struct OpMax { __device__ int operator()(int a, int b) const { return a > b ? a : b; } };
+__device__ __forceinline__ int scan_add(int x) { return dpp_scan(x, 0, OpAdd()); }
+__device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, 63); }
+
+// Resolve one (path, row) bucket of n <= 64*S fragments with one wavefront: stage the
+// records in LDS, rank them by (cell x, index), then walk the sorted order in chunks of
+// 64 carrying the winding number and item count.  Per cell group, the first lane sums
+// the members' masks into the per-sample stencil; spans open between groups whose
+// winding passes the fill rule.
+template <int SPP, int S>
+__device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, uint32_t base, int n, uint32_t *key,
+                                               uint32_t *order, uint32_t *meta, uint32_t *msk) {
     constexpr int NW = Words<SPP>::n;
     const int lane = threadIdx.x & 63;
-    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (b >= A.NB) return;
-    const uint32_t base = A.boff[b];
-    const int n = (int)(A.boff[b + 1] - base);
-    if (n == 0) {
-        if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; }
-        return;
-    }
-    if (n > 64) {
-        if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)b;
-        return;
-    }
-    const bool has = lane < n;
-    const uint32_t meta = has ? A.rec_meta[base + lane] : 0u;
-    uint32_t mk[NW];
+    uint32_t mykey[S];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) mk[k] = 0u;
-    if (has) load_mask<NW>(A.rec_mask, (size_t)base + lane, mk);
-    const uint32_t key = has ? (((meta & 0xFFFFu) << 6) | (uint32_t)lane) : 0xFFFFFFFFu;
-    int rank = 0;
-    for (int j = 0; j < n; ++j) {
-        uint32_t kj = __shfl(key, j, 64);
-        rank += (kj < key) ? 1 : 0;
-    }
-    // lane k receives the index of the lane holding the k-th smallest key
-    const int src = __builtin_amdgcn_ds_permute(rank << 2, lane);
-    const bool v = lane < n;
-    const uint32_t sm = __shfl(meta, src, 64);
-    uint32_t smk[NW];
+    for (int s = 0; s < S; ++s) {
+        const int i = s * 64 + lane;
+        mykey[s] = 0xFFFFFFFFu;
+        if (i < n) {
+            const uint32_t m = A.rec_meta[base + i];
+            meta[i] = m;
+            uint32_t mk[NW];
+            load_mask<NW>(A.rec_mask, (size_t)base + i, mk);
 #pragma unroll
-    for (int k = 0; k < NW; ++k) smk[k] = __shfl(mk[k], src, 64);
-    const int cx = (int)(sm & 0xFFFFu);
-    const int swn = (int)(int8_t)(sm >> 16);
-    const int prev_cx = __shfl_up(cx, 1, 64);
-    const bool head = v && (lane == 0 || cx != prev_cx);
-    const int excl = wave_incl_scan_i(v ? swn : 0) - (v ? swn : 0);
-    const int8_t w_at = (int8_t)(int)(A.bcarry[b] + (uint32_t)excl);
-    const int2 info = A.binfo[b];
-    const int rule = A.frule[info.x];
-    const unsigned long long hm = __ballot(head);
-    const unsigned long long above = (lane == 63) ? 0ull : (hm >> (lane + 1)) << (lane + 1);
-    const int gend = above ? (__ffsll((long long)above) - 1) : n;
-    const int gsize = head ? gend - lane : 0;
-    const int x = 2 * cx - 2;
-    const bool merged = head && x >= 0 && x < A.W;
-    uint32_t res[NW];
-    {
-        Stencil<NW> st;
-        st.init(w_at);
-        for (int j = 0;; ++j) {
-            const bool act = merged && j < gsize;
-            if (!__ballot(act)) break;
-            const int srcl = (lane + j) & 63;
-            const uint32_t mm = __shfl(sm, srcl, 64);
-            uint32_t m2[NW];
-#pragma unroll
-            for (int k = 0; k < NW; ++k) m2[k] = __shfl(smk[k], srcl, 64);
-            if (act) st.add(m2, ((mm >> 24) & 1u) != 0u);
+            for (int k = 0; k < NW; ++k) msk[i * NW + k] = mk[k];
+            mykey[s] = ((m & 0xFFFFu) << 10) | (uint32_t)i;
+            key[i] = mykey[s];
         }
-        st.finish(rule, res);
     }
-    const int prev_x = 2 * prev_cx - 2;
-    const bool span = head && lane > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
-    const int cnt = (merged ? 1 : 0) + (span ? 1 : 0);
-    const int idx = wave_incl_scan_i(cnt) - cnt;
-    const int total = __shfl(idx + cnt, 63, 64);
+    __syncthreads();
+    int rk[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) rk[s] = 0;
+    if (S == 1) {
+        for (int j = 0; j < n; ++j) {
+            const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)mykey[0], j);
+            rk[0] += (kj < mykey[0]) ? 1 : 0;
+        }
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const uint32_t kj = key[j];
+#pragma unroll
+            for (int s = 0; s < S; ++s) rk[s] += (kj < mykey[s]) ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (s * 64 + lane < n) order[rk[s]] = (uint32_t)(s * 64 + lane);
+    __syncthreads();
+
+    const int rule = A.frule[A.binfo[b].x];
+    uint32_t wcarry = A.bcarry[b];
+    int icarry = 0, nmerged = 0, xmin = INT_MAX, xmax = INT_MIN;
     const size_t ib = 2 * (size_t)base;
-    if (span) {
-        const int x0 = prev_x + 2;
-        A.items[ib + idx] = (uint32_t)x0 | ((uint32_t)(x - x0) << 16);
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+        if (c * 64 >= n) break;
+        const int p = c * 64 + lane;
+        const bool v = p < n;
+        const uint32_t i = v ? order[p] : 0u;
+        const uint32_t m = v ? meta[i] : 0u;
+        const int cx = (int)(m & 0xFFFFu);
+        const int swn = v ? (int)(int8_t)(m >> 16) : 0;
+        const int prev_cx = (v && p > 0) ? (int)(meta[order[p - 1]] & 0xFFFFu) : -1;
+        const bool head = v && (p == 0 || cx != prev_cx);
+        const int incl = scan_add(swn);
+        const int8_t w_at = (int8_t)(int)(wcarry + (uint32_t)(incl - swn));
+        wcarry += (uint32_t)lane63(incl);
+        const int x = 2 * cx - 2;
+        const bool merged = head && x >= 0 && x < A.W;
+        uint32_t res[NW];
+        if (merged) {
+            Stencil<NW> st;
+            st.init(w_at);
+            for (int q = p; q < n; ++q) {
+                const uint32_t iq = order[q];
+                const uint32_t mq = meta[iq];
+                if ((int)(mq & 0xFFFFu) != cx) break;
+                st.add(msk + iq * NW, ((mq >> 24) & 1u) != 0u);
+            }
+            st.finish(rule, res);
+        }
+        const int prev_x = 2 * prev_cx - 2;
+        const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
+        const int cnt = (merged ? 1 : 0) + (span ? 1 : 0);
+        const int ic = scan_add(cnt);
+        const int idx = icarry + ic - cnt;
+        icarry += lane63(ic);
+        if (span) {
+            const int x0 = prev_x + 2;
+            A.items[ib + idx] = (uint32_t)x0 | ((uint32_t)(x - x0) << 16);
+        }
+        if (merged) {
+            A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
+            store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
+        }
+        const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
+        const int hi = merged ? x + 2 : (span ? x : INT_MIN);
+        xmin = tmin(xmin, lane63(dpp_scan(lo, INT_MAX, OpMin())));
+        xmax = tmax(xmax, lane63(dpp_scan(hi, INT_MIN, OpMax())));
+        nmerged += lane63(scan_add(merged ? 1 : 0));
     }
-    if (merged) {
-        A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
-        store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
-    }
-    const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
-    const int hi = merged ? x + 2 : (span ? x : INT_MIN);
-    const int xmin = wave_min_i(lo), xmax = wave_max_i(hi);
-    const int nm = wave_sum_i(merged ? 1 : 0);
     if (lane == 0) {
-        A.bicnt[b] = (uint32_t)total;
-        A.bmcnt[b] = (uint32_t)nm;
-        A.bext[b] = total ? ((uint32_t)xmin | ((uint32_t)xmax << 16)) : 0xFFFFu;
+        A.bicnt[b] = (uint32_t)icarry;
+        A.bmcnt[b] = (uint32_t)nmerged;
+        A.bext[b] = icarry ? ((uint32_t)xmin | ((uint32_t)xmax << 16)) : 0xFFFFu;
+    }
+    __syncthreads();
+}
+
+constexpr int kResolveBlocks = 16384;
+
+// Every bucket: empty ones are cleared, buckets of at most 64 fragments are resolved,
+// larger ones are listed for the size-class passes below.
+template <int SPP>
+__global__ void __launch_bounds__(64) k_resolve_first(ResolveArgs A) {
+    constexpr int NW = Words<SPP>::n;
+    __shared__ uint32_t key[64], order[64], meta[64], msk[64 * NW];
+    const int lane = threadIdx.x;
+    for (int64_t b = blockIdx.x; b < A.NB; b += gridDim.x) {
+        const uint32_t base = A.boff[b];
+        const int n = (int)(A.boff[b + 1] - base);
+        if (n == 0) {
+            if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; }
+            continue;
+        }
+        if (n > 64) {
+            if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)b;
+            continue;
+        }
+        resolve_bucket<SPP, 1>(A, b, base, n, key, order, meta, msk);
     }
 }
 
-constexpr int kLdsKeys = 8192;
-
-// One wavefront per bucket of more than 64 fragments: bitonic-sort (cx << 16 | index)
-// keys in LDS (or global scratch beyond kLdsKeys), then walk the sorted order in chunks
-// of 64 carrying the winding and item counts.
-template <int SPP>
-__global__ void __launch_bounds__(64) k_resolve_large(ResolveArgs A) {
+// Buckets of 64*S/2 < n <= 64*S fragments from the list.
+template <int SPP, int S>
+__global__ void __launch_bounds__(64) k_resolve_class(ResolveArgs A) {
     constexpr int NW = Words<SPP>::n;
-    __shared__ uint32_t lkeys[kLdsKeys];
+    __shared__ uint32_t key[64 * S], order[64 * S], meta[64 * S], msk[64 * S * NW];
+    const uint32_t nbig = *A.big_count;
+    for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
+        const int64_t b = A.big_list[bi];
+        const uint32_t base = A.boff[b];
+        const int n = (int)(A.boff[b + 1] - base);
+        if (n <= 32 * S || n > 64 * S) continue;
+        resolve_bucket<SPP, S>(A, b, base, n, key, order, meta, msk);
+    }
+}
+
+// Buckets of more than 512 fragments (rare): bitonic-sort (cx << 16 | index) keys in a
+// global scratch region, then walk the sorted order in chunks of 64 reading the records
+// from global memory.
+template <int SPP>
+__global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
+    constexpr int NW = Words<SPP>::n;
     const int lane = threadIdx.x;
     const uint32_t nbig = *A.big_count;
     for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
         const int64_t b = A.big_list[bi];
         const uint32_t base = A.boff[b];
         const int n = (int)(A.boff[b + 1] - base);
+        if (n <= 512) continue;
         if (n > 65536) {
             if (lane == 0) { atomicOr(A.err, 2u); A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; }
             continue;
         }
         int P2 = 1;
         while (P2 < n) P2 <<= 1;
-        uint32_t *keys = (P2 <= kLdsKeys) ? lkeys : (A.scratch + 2 * (size_t)base);
+        uint32_t *keys = A.scratch + 2 * (size_t)base;
         for (int i = lane; i < P2; i += 64)
             keys[i] = (i < n) ? (((A.rec_meta[base + i] & 0xFFFFu) << 16) | (uint32_t)i) : 0xFFFFFFFFu;
         __syncthreads();
@@ -255,23 +326,22 @@ __global__ void __launch_bounds__(64) k_resolve_large(ResolveArgs A) {
                 __syncthreads();
             }
         }
-        const int2 info = A.binfo[b];
-        const int rule = A.frule[info.x];
+        const int rule = A.frule[A.binfo[b].x];
         uint32_t wcarry = A.bcarry[b];
         int icarry = 0, nmerged = 0, xmin = INT_MAX, xmax = INT_MIN;
         const size_t ib = 2 * (size_t)base;
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int p = c0 + lane;
             const bool v = p < n;
-            const uint32_t key = v ? keys[p] : 0xFFFFFFFFu;
-            const int cx = (int)(key >> 16);
-            const uint32_t meta = v ? A.rec_meta[base + (key & 0xFFFFu)] : 0u;
+            const uint32_t kk = v ? keys[p] : 0xFFFFFFFFu;
+            const int cx = (int)(kk >> 16);
+            const uint32_t meta = v ? A.rec_meta[base + (kk & 0xFFFFu)] : 0u;
             const int swn = v ? (int)(int8_t)(meta >> 16) : 0;
             const int prev_cx = (v && p > 0) ? (int)(keys[p - 1] >> 16) : -1;
             const bool head = v && (p == 0 || cx != prev_cx);
-            const int incl = wave_incl_scan_i(swn);
+            const int incl = scan_add(swn);
             const int8_t w_at = (int8_t)(int)(wcarry + (uint32_t)(incl - swn));
-            wcarry += (uint32_t)__shfl(incl, 63, 64);
+            wcarry += (uint32_t)lane63(incl);
             const int x = 2 * cx - 2;
             const bool merged = head && x >= 0 && x < A.W;
             uint32_t res[NW];
@@ -291,9 +361,9 @@ __global__ void __launch_bounds__(64) k_resolve_large(ResolveArgs A) {
             const int prev_x = 2 * prev_cx - 2;
             const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
             const int cnt = (merged ? 1 : 0) + (span ? 1 : 0);
-            const int ic = wave_incl_scan_i(cnt);
+            const int ic = scan_add(cnt);
             const int idx = icarry + ic - cnt;
-            icarry += __shfl(ic, 63, 64);
+            icarry += lane63(ic);
             if (span) {
                 const int x0 = prev_x + 2;
                 A.items[ib + idx] = (uint32_t)x0 | ((uint32_t)(x - x0) << 16);
@@ -304,9 +374,9 @@ __global__ void __launch_bounds__(64) k_resolve_large(ResolveArgs A) {
             }
             const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
             const int hi = merged ? x + 2 : (span ? x : INT_MIN);
-            xmin = tmin(xmin, wave_min_i(lo));
-            xmax = tmax(xmax, wave_max_i(hi));
-            nmerged += wave_sum_i(merged ? 1 : 0);
+            xmin = tmin(xmin, lane63(dpp_scan(lo, INT_MAX, OpMin())));
+            xmax = tmax(xmax, lane63(dpp_scan(hi, INT_MIN, OpMax())));
+            nmerged += lane63(scan_add(merged ? 1 : 0));
         }
         if (lane == 0) {
             A.bicnt[b] = (uint32_t)icarry;
@@ -656,18 +726,20 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
     A.items = f.items; A.imask = f.imask; A.bicnt = f.bicnt; A.bmcnt = f.bmcnt; A.bext = f.bext;
     A.big_list = f.big_list; A.big_count = f.big_count; A.err = f.err_flag;
     launch_fill_u32(f.big_count, 0u, 1, s);
-    dim3 gs(blocks_for(f.NB, 4)), bs(256);
-    dim3 gl(1024), bl(64);
-    if (t.spp == 8) {
-        hipLaunchKernelGGL(k_resolve_small<8>, gs, bs, 0, s, A);
-        hipLaunchKernelGGL(k_resolve_large<8>, gl, bl, 0, s, A);
-    } else if (t.spp == 16) {
-        hipLaunchKernelGGL(k_resolve_small<16>, gs, bs, 0, s, A);
-        hipLaunchKernelGGL(k_resolve_large<16>, gl, bl, 0, s, A);
-    } else {
-        hipLaunchKernelGGL(k_resolve_small<32>, gs, bs, 0, s, A);
-        hipLaunchKernelGGL(k_resolve_large<32>, gl, bl, 0, s, A);
-    }
+    const unsigned g1 = (unsigned)std::min<int64_t>(f.NB, kResolveBlocks);
+    dim3 b64(64);
+#define SL_LAUNCH(N)                                                                  \
+    do {                                                                              \
+        hipLaunchKernelGGL(k_resolve_first<N>, dim3(g1), b64, 0, s, A);               \
+        hipLaunchKernelGGL((k_resolve_class<N, 2>), dim3(4096), b64, 0, s, A);        \
+        hipLaunchKernelGGL((k_resolve_class<N, 4>), dim3(4096), b64, 0, s, A);        \
+        hipLaunchKernelGGL((k_resolve_class<N, 8>), dim3(4096), b64, 0, s, A);        \
+        hipLaunchKernelGGL(k_resolve_huge<N>, dim3(1024), b64, 0, s, A);              \
+    } while (0)
+    if (t.spp == 8) SL_LAUNCH(8);
+    else if (t.spp == 16) SL_LAUNCH(16);
+    else SL_LAUNCH(32);
+#undef SL_LAUNCH
 }
 
 void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s) {

@@ -139,3 +139,18 @@ def test_empty_and_invisible(rasterizer):
     cnt, exact = compare(rasterizer, vg, 64, 64, 32, label="invisible")
     assert cnt["curve_fragment"] == 0
     assert (rasterizer.output() == 255).all()
+
+
+@pytest.mark.parametrize("length", [120, 230, 460, 1900])
+def test_bucket_size_classes(rasterizer, length):
+    """Thin slivers lying inside one cell row put ~length fragments into a single
+    (path, row) bucket, covering every resolve size class (<=64, <=128, <=256, <=512 and
+    the global-scratch path beyond)."""
+    vg = VGContainer()
+    vg.addPath(0, rgba(200, 30, 30, 200))
+    vg.addPolygon([(4.3, 10.2), (4.3 + length, 10.7), (4.3 + length, 11.6), (4.3, 11.1)])
+    vg.addPath(1, rgba(30, 30, 200, 150))
+    vg.addPolygon([(6.1, 12.1), (6.1 + length, 12.2), (6.1 + length / 2, 13.9)])
+    vg.addPolygon([(9.1, 12.3), (9.1 + length / 3, 12.5), (9.1 + length / 4, 13.7)])
+    for spp in (8, 32):
+        compare(rasterizer, vg, 2048, 24, spp, label="sliver%d" % length)
