@@ -61,9 +61,9 @@ struct sl_handle {
     // per-frame arena
     DevBuf b_pcnt, b_nchain, b_choff, b_chains, b_chain_t;
     DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
-        b_boff, b_bwsum, b_bcarry, b_bcursor, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_bicnt, b_bmcnt,
+        b_boff, b_bwsum, b_bcarry, b_bcursor, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
         b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
-        b_rowbeg, b_rowend, b_image, b_scan_tmp;
+        b_rowbeg, b_rowend, b_rrec, b_image, b_scan_tmp;
     // debug host copies
     std::vector<int32_t> d_ts, d_yx, d_pid, d_draw;
     std::vector<int8_t> d_wn;
@@ -134,6 +134,7 @@ int spp_slot(int spp) { return spp == 8 ? 0 : (spp == 16 ? 1 : 2); }
 
 // colour table block layout in one device buffer
 struct ColorBlock {
+    float step_c[256][4];  // {thr[k+1], dec[k], dec[k+1], 0} of the colour encoding
     float dec_c[256], thr_c[256], dec_a[256], thr_a[256];
     uint8_t lut_c[4096], lut_a[4096];
 };
@@ -158,6 +159,12 @@ int ensure_tables(sl_handle *h) {
         std::memcpy(blk.dec_a, a.dec, sizeof blk.dec_a);
         std::memcpy(blk.thr_a, a.thr, sizeof blk.thr_a);
         std::memcpy(blk.lut_a, a.lut, sizeof blk.lut_a);
+        for (int k = 0; k < 256; ++k) {
+            blk.step_c[k][0] = k < 255 ? c.thr[k + 1] : INFINITY;
+            blk.step_c[k][1] = c.dec[k];
+            blk.step_c[k][2] = c.dec[k < 255 ? k + 1 : 255];
+            blk.step_c[k][3] = 0.f;
+        }
         int rc = upload(h, ct, &blk, sizeof blk);
         if (rc) return rc;
     }
@@ -172,6 +179,8 @@ sl::Tables tables_view(sl_handle *h) {
     ColorBlock *cb = P_<ColorBlock>(h->ctab[h->srgb ? 1 : 0]);
     t.dec_c = cb->dec_c; t.thr_c = cb->thr_c; t.lut_c = cb->lut_c;
     t.dec_a = cb->dec_a; t.thr_a = cb->thr_a; t.lut_a = cb->lut_a;
+    t.step_c = (const float4 *)cb->step_c;
+    t.srgb = h->srgb;
     return t;
 }
 
@@ -294,25 +303,28 @@ int render(sl_handle *h, hipStream_t s) {
         (rc = ensure(h, h->b_bcarry, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bcursor, (size_t)NB * 4)) ||
         (rc = ensure(h, h->b_rec_meta, (size_t)F * 4)) || (rc = ensure(h, h->b_rec_mask, (size_t)F * 4 * NW)) ||
         (rc = ensure(h, h->b_scratch, (size_t)F * 8)) || (rc = ensure(h, h->b_items, (size_t)F * 8)) ||
-        (rc = ensure(h, h->b_imask, (size_t)F * 8 * NW)) || (rc = ensure(h, h->b_bicnt, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure(h, h->b_imask, (size_t)F * 8 * NW)) || (rc = ensure(h, h->b_icov, (size_t)F * 8)) || (rc = ensure(h, h->b_bicnt, ((size_t)NB + 1) * 4)) ||
         (rc = ensure(h, h->b_bmcnt, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bext, (size_t)NB * 4)) ||
         (rc = ensure(h, h->b_big_list, (size_t)NB * 4)) || (rc = ensure(h, h->b_nz_flag, ((size_t)NB + 1) * 4)) ||
         (rc = ensure(h, h->b_nz_off, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_nzk0, (size_t)NB * 4)) ||
         (rc = ensure(h, h->b_nzk1, (size_t)NB * 4)) || (rc = ensure(h, h->b_nzv0, (size_t)NB * 4)) ||
         (rc = ensure(h, h->b_nzv1, (size_t)NB * 4)) || (rc = ensure(h, h->b_rs_hist, 256 * rs_tiles * 4 + 16)) ||
         (rc = ensure(h, h->b_rs_hoff, (256 * rs_tiles + 1) * 4 + 16)) || (rc = ensure(h, h->b_rowbeg, R * 4)) ||
-        (rc = ensure(h, h->b_rowend, R * 4)) || (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
+        (rc = ensure(h, h->b_rowend, R * 4)) || (rc = ensure(h, h->b_rrec, (size_t)NB * 16)) ||
+        (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
         return rc;
     f.binfo = P_<int2>(h->b_binfo); f.bcount = P_<uint32_t>(h->b_bcount); f.boff = P_<uint32_t>(h->b_boff);
     f.bwsum = P_<uint32_t>(h->b_bwsum); f.bcarry = P_<uint32_t>(h->b_bcarry); f.bcursor = P_<uint32_t>(h->b_bcursor);
     f.rec_meta = P_<uint32_t>(h->b_rec_meta); f.rec_mask = P_<uint32_t>(h->b_rec_mask);
     f.scratch_keys = P_<uint32_t>(h->b_scratch); f.items = P_<uint32_t>(h->b_items); f.imask = P_<uint32_t>(h->b_imask);
+    f.icov = P_<uint32_t>(h->b_icov);
     f.bicnt = P_<uint32_t>(h->b_bicnt); f.bmcnt = P_<uint32_t>(h->b_bmcnt); f.bext = P_<uint32_t>(h->b_bext);
     f.big_list = P_<uint32_t>(h->b_big_list); f.nz_flag = P_<uint32_t>(h->b_nz_flag); f.nz_off = P_<uint32_t>(h->b_nz_off);
     f.nz_keys[0] = P_<uint32_t>(h->b_nzk0); f.nz_keys[1] = P_<uint32_t>(h->b_nzk1);
     f.nz_vals[0] = P_<uint32_t>(h->b_nzv0); f.nz_vals[1] = P_<uint32_t>(h->b_nzv1);
     f.rs_hist = P_<uint32_t>(h->b_rs_hist); f.rs_hoff = P_<uint32_t>(h->b_rs_hoff);
     f.rowbeg = P_<uint32_t>(h->b_rowbeg); f.rowend = P_<uint32_t>(h->b_rowend);
+    f.rrec = P_<uint4>(h->b_rrec);
     f.scan_tmp = P_<uint32_t>(h->b_scan_tmp);
 
     sl::launch_bucket_info(sc, f, s);
@@ -456,7 +468,7 @@ int sl_destroy(sl_handle *h) {
                       &h->ptable[0], &h->ptable[1], &h->ptable[2], &h->ctab[0], &h->ctab[1], &h->b_tvtx, &h->b_pflags,
                       &h->b_prow, &h->b_ccuts, &h->b_cncut, &h->b_cfrag, &h->b_coff, &h->b_ts, &h->b_pnb, &h->b_pbase,
                       &h->b_binfo, &h->b_bcount, &h->b_boff, &h->b_bwsum, &h->b_bcarry, &h->b_bcursor, &h->b_rec_meta,
-                      &h->b_rec_mask, &h->b_scratch, &h->b_items, &h->b_imask, &h->b_bicnt, &h->b_bmcnt, &h->b_bext,
+                      &h->b_rec_mask, &h->b_scratch, &h->b_items, &h->b_imask, &h->b_icov, &h->b_bicnt, &h->b_bmcnt, &h->b_bext,
                       &h->b_big_list, &h->b_small, &h->b_nz_flag, &h->b_nz_off, &h->b_nzk0, &h->b_nzk1, &h->b_nzv0,
                       &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp};
     for (DevBuf *b : bufs) release(*b);
@@ -637,6 +649,52 @@ const char *sl_last_error(sl_handle *h) { return h ? h->err.c_str() : "null hand
 int sl_debug_enable(sl_handle *h, int enable) {
     if (!h) return SL_E_ARG;
     h->debug = enable != 0;
+    return SL_OK;
+}
+
+int sl_debug_requantize(sl_handle *h, int channel, const float *values, int64_t n, uint8_t *codes, float *decoded) {
+    if (!h || (channel != 0 && channel != 1) || n < 0 || (n && (!values || !codes || !decoded))) return SL_E_ARG;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    int rc;
+    if ((rc = ensure_tables(h))) return rc;
+    if (!n) return SL_OK;
+    const sl::Tables tb = tables_view(h);
+    void *dv = nullptr, *dc = nullptr, *dd = nullptr;
+    hipError_t e = hipMalloc(&dv, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMalloc(&dc, (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&dd, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMemcpy(dv, values, (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        sl::launch_encode_values(tb, channel, (const float *)dv, n, (uint8_t *)dc, (float *)dd, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(codes, dc, (size_t)n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(decoded, dd, (size_t)n * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(dv);
+    (void)hipFree(dc);
+    (void)hipFree(dd);
+    if (e != hipSuccess) return fail(h, SL_E_HIP, "requantize: %s", hipGetErrorString(e));
+    return SL_OK;
+}
+
+int sl_debug_requantize_sweep(sl_handle *h, int channel, uint32_t bits_lo, uint32_t bits_hi, uint64_t *mismatches) {
+    if (!h || (channel != 0 && channel != 1) || !mismatches) return SL_E_ARG;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    int rc;
+    if ((rc = ensure_tables(h))) return rc;
+    const sl::Tables tb = tables_view(h);
+    void *db = nullptr;
+    unsigned long long bad = 0;
+    hipError_t e = hipMalloc(&db, 8);
+    if (e == hipSuccess) e = hipMemset(db, 0, 8);
+    if (e == hipSuccess) {
+        sl::launch_encode_sweep(tb, channel, bits_lo, bits_hi, (unsigned long long *)db, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&bad, db, 8, hipMemcpyDeviceToHost);
+    (void)hipFree(db);
+    if (e != hipSuccess) return fail(h, SL_E_HIP, "requantize sweep: %s", hipGetErrorString(e));
+    *mismatches = bad;
     return SL_OK;
 }
 

@@ -24,6 +24,7 @@ struct SceneDev {
 
 struct Tables {
     int spp;                 // 8, 16 or 32
+    int srgb;                // colour channels sRGB-encoded
     int words;               // mask words per fragment: 1, 2, 4
     const uint32_t *pixel;   // [65536 * words] line -> sample mask table
     const float *dec_c;      // [256] colour decode (sRGB or UNORM)
@@ -32,6 +33,7 @@ struct Tables {
     const float *dec_a;      // [256] alpha decode (UNORM)
     const float *thr_a;      // [256]
     const uint8_t *lut_a;    // [4096]
+    const float4 *step_c;    // [256] sRGB steps {thr[k+1], dec[k], dec[k+1], 0}
 };
 
 struct Frame {
@@ -69,6 +71,7 @@ struct Frame {
     // resolve outputs, per bucket and per item slot (2 slots per valid fragment)
     uint32_t *items;         // [2F]
     uint32_t *imask;         // [2F * words]
+    uint32_t *icov;          // [2F] per merged item: sample counts of its 4 pixels
     uint32_t *bicnt;         // [NB+1]
     uint32_t *bmcnt;         // [NB+1]
     uint32_t *bext;          // [NB]
@@ -84,6 +87,7 @@ struct Frame {
     uint32_t *rs_hoff;
     uint32_t *rowbeg;        // [R]
     uint32_t *rowend;        // [R]
+    uint4 *rrec;             // [NB] row-list records {extent, colour, items, item base}
     uint8_t *image;          // [H * pitch]
     int pitch;
     int srgb, ms;
@@ -107,6 +111,10 @@ void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStrea
 void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
 void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
+void launch_encode_values(const Tables &t, int channel, const float *v, int64_t n, uint8_t *code, float *dec,
+                          hipStream_t s);
+void launch_encode_sweep(const Tables &t, int channel, uint32_t lo, uint32_t hi, unsigned long long *bad,
+                         hipStream_t s);
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 

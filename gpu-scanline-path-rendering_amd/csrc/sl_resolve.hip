@@ -94,6 +94,21 @@ struct Stencil {
     }
 };
 
+// Sample counts of the four pixels of a resolved cell mask, 6 bits each at bit 6*(dx*2+dy).
+template <int SPP>
+__device__ __forceinline__ uint32_t cell_counts(const uint32_t *m) {
+    uint32_t c = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t bits;
+        if (SPP == 8) bits = (m[0] >> (q * 8)) & 0xFFu;
+        else if (SPP == 16) bits = (m[q >> 1] >> ((q & 1) * 16)) & 0xFFFFu;
+        else bits = m[q];
+        c |= (uint32_t)__popc(bits) << (6 * q);
+    }
+    return c;
+}
+
 template <int NW>
 __device__ __forceinline__ void store_mask(uint32_t *imask, size_t slot, const uint32_t *m) {
     if (NW == 4) *(uint4 *)(imask + slot * 4) = make_uint4(m[0], m[1], m[2], m[3]);
@@ -119,6 +134,7 @@ struct ResolveArgs {
     int W;
     uint32_t *items;
     uint32_t *imask;
+    uint32_t *icov;
     uint32_t *bicnt;
     uint32_t *bmcnt;
     uint32_t *bext;
@@ -236,6 +252,7 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
         if (merged) {
             A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
             store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
+            A.icov[ib + idx + (span ? 1 : 0)] = cell_counts<SPP>(res);
         }
         const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
         const int hi = merged ? x + 2 : (span ? x : INT_MIN);
@@ -371,6 +388,7 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
             if (merged) {
                 A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
                 store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
+                A.icov[ib + idx + (span ? 1 : 0)] = cell_counts<SPP>(res);
             }
             const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
             const int hi = merged ? x + 2 : (span ? x : INT_MIN);
@@ -496,24 +514,27 @@ struct CompArgs {
     int W, H, tiles_x;
     const uint32_t *rowbeg, *rowend, *rowvals;
     const int2 *binfo;
-    const uint32_t *finfo, *bicnt, *boff, *bext, *items, *imask;
+    const uint32_t *finfo, *bicnt, *boff, *bext, *items, *imask, *icov;
     const float *dec_c, *thr_c, *dec_a, *thr_a;
     const uint8_t *lut_c, *lut_a;
     uint8_t *image;
     int pitch;
 };
 
+// thr_* hold 257 entries: thr[256] = +inf ends the threshold bump at code 255.
 struct LdsTables {
-    float dec_c[256], thr_c[256], dec_a[256], thr_a[256];
+    float dec_c[256], thr_c[260], dec_a[256], thr_a[260];
     uint8_t lut_c[4096], lut_a[4096];
 };
 
+// 8-bit code of a blended value: largest k with v >= thr[k] (thr[0] = -inf).  At most one
+// threshold lies in each 1/4096 interval, so the LUT entry of v's interval is either the
+// code or one below it.  Branch-free; negative values and NaN give 0, v >= 1 gives 255.
 __device__ __forceinline__ uint32_t enc8(const float *thr, const uint8_t *lut, float v) {
-    if (!(v >= 0.f)) return 0u;
-    if (v >= 1.f) return 255u;
-    uint32_t k = lut[(int)(v * 4096.f)];
-    if (k < 255u && v >= thr[k + 1]) ++k;
-    return k;
+    const float fi = v * 4096.f;
+    const int i = (fi >= 0.f) ? ((fi < 4095.f) ? (int)fi : 4095) : 0;
+    const uint32_t k = lut[i];
+    return k + ((v >= thr[k + 1]) ? 1u : 0u);
 }
 
 __device__ __forceinline__ uint32_t blend_px(const LdsTables &T, uint32_t dst, const float *sc, float a) {
@@ -536,103 +557,201 @@ __device__ __forceinline__ uint32_t pixel_bits(const uint32_t *m, int dx, int dy
     return m[dx * 2 + dy];
 }
 
+constexpr int kListCap = 256;  // per-wave list of overlapping buckets
+constexpr int kGroup = 8;      // buckets whose items are loaded together
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Blend one bucket's coverage of this lane's two pixels.  `bits0/bits1` are the sample
+// masks (multisample mode) and `n0/n1` the sample counts (coverage mode).
+template <int SPP, bool MS>
+__device__ __forceinline__ void blend_cov(const LdsTables &T, uint32_t *st0, uint32_t *st1, uint32_t c, uint32_t n0,
+                                          uint32_t n1, uint32_t bits0, uint32_t bits1) {
+    constexpr int NS = MS ? SPP : 1;
+    float sc[3];
+    sc[0] = T.dec_c[c & 0xFFu];
+    sc[1] = T.dec_c[(c >> 8) & 0xFFu];
+    sc[2] = T.dec_c[(c >> 16) & 0xFFu];
+    const float sa = T.dec_a[c >> 24];
+    if (!MS) {
+        st0[0] = blend_px(T, st0[0], sc, sa * ((float)n0 / (float)SPP));
+        st1[0] = blend_px(T, st1[0], sc, sa * ((float)n1 / (float)SPP));
+    } else {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if ((bits0 >> s) & 1u) st0[s] = blend_px(T, st0[s], sc, sa);
+            if ((bits1 >> s) & 1u) st1[s] = blend_px(T, st1[s], sc, sa);
+        }
+    }
+}
+
+// Items of one bucket are disjoint and sorted by x: the covering item of column x is the
+// last one with x0 <= x, found by a lane-parallel binary search over the 64 loaded items.
+__device__ __forceinline__ int find_item(uint32_t it, int m, int x) {
+    const int ix0 = (int)(it & 0xFFFFu);
+    int pos = -1;
+#pragma unroll
+    for (int step = 64; step >= 1; step >>= 1) {
+        const int cand = pos + step;
+        const int cx0 = __shfl(ix0, cand < 64 ? cand : 63, 64);
+        if (cand < m && cx0 <= x) pos = cand;
+    }
+    return pos;
+}
+
+template <int SPP, bool MS>
+__device__ __noinline__ void blend_bucket_chunked(const CompArgs &A, const LdsTables &T, uint32_t *st0, uint32_t *st1,
+                                                     uint32_t c, int cnt, uint32_t ib, int tx0, int x) {
+    constexpr int NW = Words<SPP>::n;
+    const int lane = threadIdx.x & 63;
+    for (int ic = 0; ic < cnt; ic += 64) {
+        const int m = tmin(64, cnt - ic);
+        const uint32_t it = lane < m ? A.items[ib + ic + lane] : 0u;
+        const int ix0 = (int)(it & 0xFFFFu), iw = (int)(it >> 16);
+        const int iend = ix0 + (iw ? iw : 2);
+        if (__shfl(iend, m - 1, 64) <= tx0) continue;
+        if (__shfl(ix0, 0, 64) >= tx0 + 64) break;
+        const int pos = find_item(it, m, x);
+        const int p = pos < 0 ? 0 : pos;
+        const int pend = __shfl(iend, p, 64), pw = __shfl(iw, p, 64), px0 = __shfl(ix0, p, 64);
+        if (!(pos >= 0 && x < pend && x < A.W)) continue;
+        uint32_t n0 = SPP, n1 = SPP, b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu;
+        if (pw == 0) {
+            if (!MS) {
+                const uint32_t cv = A.icov[ib + ic + pos];
+                n0 = (cv >> (6 * ((x - px0) * 2))) & 63u;
+                n1 = (cv >> (6 * ((x - px0) * 2 + 1))) & 63u;
+            } else {
+                uint32_t mk[NW];
+                load_mask<NW>(A.imask, (size_t)ib + ic + pos, mk);
+                b0 = pixel_bits<SPP>(mk, x - px0, 0);
+                b1 = pixel_bits<SPP>(mk, x - px0, 1);
+            }
+        }
+        blend_cov<SPP, MS>(T, st0, st1, c, n0, n1, b0, b1);
+    }
+}
+
+// One wavefront per 64x2 pixel tile (lane = pixel column, rows 2r and 2r+1).  The row's
+// non-empty buckets are in path order; the wave first filters those overlapping the tile
+// into an LDS list (colour, item count, item base), then walks the list in groups of
+// kGroup buckets: it issues all the group's item loads together, and per bucket (in path
+// order) finds each column's covering item and blends it.  Blending is src-over into an
+// 8-bit framebuffer re-quantised after every blend: coverage alpha = paint alpha *
+// popcount(mask) / N (output_{8,32}.frag.glsl), or per-sample blending plus a resolve in
+// multisample mode (ms_output_{8,32}.frag.glsl).
 template <int SPP, bool MS>
 __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
     constexpr int NW = Words<SPP>::n;
-    constexpr uint32_t FULL = SPP == 32 ? 0xFFFFFFFFu : ((1u << SPP) - 1u);
+    constexpr uint32_t FULLB = SPP == 32 ? 0xFFFFFFFFu : ((1u << SPP) - 1u);
     __shared__ LdsTables T;
+    __shared__ uint32_t lcol[4][kListCap], lcnt[4][kListCap], lib[4][kListCap];
+    __shared__ uint32_t gitem[4][kGroup][64], gcv[4][kGroup][MS ? 1 : 64];
     for (int i = threadIdx.x; i < 256; i += 256) {
         T.dec_c[i] = A.dec_c[i]; T.thr_c[i] = A.thr_c[i];
         T.dec_a[i] = A.dec_a[i]; T.thr_a[i] = A.thr_a[i];
+    }
+    if (threadIdx.x < 4) {
+        T.thr_c[256 + threadIdx.x] = __builtin_inff();
+        T.thr_a[256 + threadIdx.x] = __builtin_inff();
     }
     for (int i = threadIdx.x; i < 4096 / 4; i += 256) {
         ((uint32_t *)T.lut_c)[i] = ((const uint32_t *)A.lut_c)[i];
         ((uint32_t *)T.lut_a)[i] = ((const uint32_t *)A.lut_a)[i];
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tile = blockIdx.x * 4 + w;
     const int R = (A.H + 1) >> 1;
     if (tile >= R * A.tiles_x) return;
     const int r = tile / A.tiles_x;
     const int tx0 = (tile - r * A.tiles_x) * 64;
+    const int tx1 = tmin(tx0 + 64, A.W);
     const int x = tx0 + lane;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     constexpr int NS = MS ? SPP : 1;
     uint32_t st0[NS], st1[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) { st0[s] = 0xFFFFFFFFu; st1[s] = 0xFFFFFFFFu; }
 
+    auto flush = [&](int nlist) {
+        wave_sync_lds();
+        for (int g0 = 0; g0 < nlist; g0 += kGroup) {
+            const int ng = tmin(kGroup, nlist - g0);
+            // issue the group's item loads together, park them in LDS
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                if (g < ng) {
+                    const int cnt = (int)lcnt[w][g0 + g];
+                    const uint32_t ib = lib[w][g0 + g];
+                    const bool ld = cnt <= 64 && lane < cnt;
+                    gitem[w][g][lane] = ld ? A.items[ib + lane] : 0u;
+                    if (!MS) gcv[w][g][lane] = ld ? A.icov[ib + lane] : 0u;
+                }
+            }
+            wave_sync_lds();
+            for (int g = 0; g < ng; ++g) {
+                const uint32_t c = lcol[w][g0 + g];
+                const int cnt = (int)lcnt[w][g0 + g];
+                const uint32_t ib = lib[w][g0 + g];
+                if (cnt > 64) {
+                    blend_bucket_chunked<SPP, MS>(A, T, st0, st1, c, cnt, ib, tx0, x);
+                    continue;
+                }
+                const uint32_t it = gitem[w][g][lane];
+                const int pos = find_item(it, cnt, x);
+                const int p = pos < 0 ? 0 : pos;
+                const int ix0 = (int)(it & 0xFFFFu), iw = (int)(it >> 16);
+                const int iend = ix0 + (iw ? iw : 2);
+                const int pend = __shfl(iend, p, 64), pw = __shfl(iw, p, 64), px0 = __shfl(ix0, p, 64);
+                if (!(pos >= 0 && x < pend && x < A.W)) continue;
+                uint32_t n0 = SPP, n1 = SPP, b0 = FULLB, b1 = FULLB;
+                if (pw == 0) {
+                    if (!MS) {
+                        const uint32_t pcv = gcv[w][g][p];
+                        n0 = (pcv >> (6 * ((x - px0) * 2))) & 63u;
+                        n1 = (pcv >> (6 * ((x - px0) * 2 + 1))) & 63u;
+                    } else {
+                        uint32_t mk[NW];
+                        load_mask<NW>(A.imask, (size_t)ib + pos, mk);
+                        b0 = pixel_bits<SPP>(mk, x - px0, 0);
+                        b1 = pixel_bits<SPP>(mk, x - px0, 1);
+                    }
+                }
+                blend_cov<SPP, MS>(T, st0, st1, c, n0, n1, b0, b1);
+            }
+            wave_sync_lds();
+        }
+    };
+
     const uint32_t q0 = A.rowbeg[r], q1 = A.rowend[r];
+    int nlist = 0;
     for (uint32_t qc = q0; qc < q1; qc += 64) {
         const uint32_t q = qc + lane;
         const bool inq = q < q1;
         const uint32_t bb = inq ? A.rowvals[q] : 0u;
         const uint32_t ext = inq ? A.bext[bb] : 0xFFFFu;
         const int exlo = (int)(ext & 0xFFFFu), exhi = (int)(ext >> 16);
-        const bool ov = inq && exlo < tx0 + 64 && exhi > tx0;
-        unsigned long long om = __ballot(ov);
-        uint32_t col = 0u, icnt = 0u, ibase = 0u;
+        const bool ov = inq && exlo < tx1 && exhi > tx0;
+        const unsigned long long om = __ballot(ov);
+        if (!om) continue;
+        if (nlist + 64 > kListCap) {
+            flush(nlist);
+            nlist = 0;
+        }
         if (ov) {
-            col = A.finfo[A.binfo[bb].x];
-            icnt = A.bicnt[bb];
-            ibase = 2u * A.boff[bb];
+            const int slot = nlist + (int)__popcll(om & lt);
+            lcol[w][slot] = A.finfo[A.binfo[bb].x];
+            lcnt[w][slot] = A.bicnt[bb];
+            lib[w][slot] = 2u * A.boff[bb];
         }
-        while (om) {
-            const int j = __ffsll((long long)om) - 1;
-            om &= om - 1;
-            const uint32_t c = __shfl(col, j, 64);
-            const int cnt = (int)__shfl(icnt, j, 64);
-            const uint32_t ib = __shfl(ibase, j, 64);
-            float sc[3];
-            sc[0] = T.dec_c[c & 0xFFu];
-            sc[1] = T.dec_c[(c >> 8) & 0xFFu];
-            sc[2] = T.dec_c[(c >> 16) & 0xFFu];
-            const float sa = T.dec_a[c >> 24];
-            for (int ic = 0; ic < cnt; ic += 64) {
-                const int m = tmin(64, cnt - ic);
-                const uint32_t it = lane < m ? A.items[ib + ic + lane] : 0u;
-                const int ix0 = (int)(it & 0xFFFFu);
-                const int iw = (int)(it >> 16);
-                const int iend = ix0 + (iw ? iw : 2);
-                const int first = __shfl(ix0, 0, 64);
-                const int lastend = __shfl(iend, m - 1, 64);
-                if (lastend <= tx0) continue;
-                if (first >= tx0 + 64) break;
-                // last item with x0 <= x (items are disjoint and sorted)
-                int pos = -1;
-#pragma unroll
-                for (int step = 64; step >= 1; step >>= 1) {
-                    const int cand = pos + step;
-                    const int cx0 = __shfl(ix0, cand < 64 ? cand : 63, 64);
-                    if (cand < m && cx0 <= x) pos = cand;
-                }
-                const int pend = __shfl(iend, pos < 0 ? 0 : pos, 64);
-                const int pw = __shfl(iw, pos < 0 ? 0 : pos, 64);
-                const int px0 = __shfl(ix0, pos < 0 ? 0 : pos, 64);
-                const bool cov = pos >= 0 && x < pend && x < A.W;
-                if (!cov) continue;
-                uint32_t b0 = FULL, b1 = FULL;
-                if (pw == 0) {
-                    uint32_t mk[NW];
-                    load_mask<NW>(A.imask, (size_t)ib + ic + pos, mk);
-                    b0 = pixel_bits<SPP>(mk, x - px0, 0);
-                    b1 = pixel_bits<SPP>(mk, x - px0, 1);
-                }
-                if (!MS) {
-                    const float a0 = sa * ((float)__popc(b0) / (float)SPP);
-                    const float a1 = sa * ((float)__popc(b1) / (float)SPP);
-                    st0[0] = blend_px(T, st0[0], sc, a0);
-                    st1[0] = blend_px(T, st1[0], sc, a1);
-                } else {
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if ((b0 >> s) & 1u) st0[s] = blend_px(T, st0[s], sc, sa);
-                        if ((b1 >> s) & 1u) st1[s] = blend_px(T, st1[s], sc, sa);
-                    }
-                }
-            }
-        }
+        nlist += (int)__popcll(om);
     }
+    if (nlist) flush(nlist);
+
     if (x >= A.W) return;
     uint32_t o0, o1;
     if (!MS) {
@@ -661,6 +780,286 @@ __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
     const int y0 = 2 * r;
     *(uint32_t *)(A.image + (size_t)y0 * A.pitch + (size_t)x * 4) = o0;
     if (y0 + 1 < A.H) *(uint32_t *)(A.image + (size_t)(y0 + 1) * A.pitch + (size_t)x * 4) = o1;
+}
+
+// ---------------------------------------------------------------------------------
+// coverage-mode compositor.  The framebuffer state of each pixel is carried as the four
+// decoded floats of its 8-bit codes; every blend re-quantises to a code and decodes it
+// again, exactly as an 8-bit framebuffer does:
+//   UNORM8: code = largest k with v >= (k - 0.5) / 255, i.e. with 510 v - (2k - 1) >= 0 --
+//           one fma gives that sign exactly; decode = RN(k / 255) by a corrected product.
+//   sRGB8:  an estimate k0 of the code from the transfer function (hardware log2/exp2;
+//           its error is far below the 0.25 code units the bracketing needs) leaves the
+//           code at k0 or k0 + 1; one 16-byte step entry {thr[k0+1], dec[k0], dec[k0+1]}
+//           decides and decodes.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int unorm_code(float v) {
+    int k = tmin(tmax((int)__builtin_fmaf(v, 255.f, 0.5f), 0), 255);
+    const bool below = __builtin_fmaf(v, 510.f, (float)(1 - 2 * k)) < 0.f;        // v < thr[k]
+    const bool above = __builtin_fmaf(v, 510.f, (float)(-1 - 2 * k)) >= 0.f;      // v >= thr[k+1]
+    k += (above && k < 255) ? 1 : 0;
+    k -= below ? 1 : 0;
+    return tmax(k, 0);
+}
+
+__device__ __forceinline__ float unorm_dec(int k) {
+    const float fk = (float)k, r = 1.f / 255.f;
+    const float q = fk * r;
+    return __builtin_fmaf(__builtin_fmaf(-q, 255.f, fk), r, q);
+}
+
+__device__ __forceinline__ int srgb_k0(float v) {
+    const float e = v <= 0.0031308f
+                        ? v * (12.92f * 255.f)
+                        : __builtin_fmaf(__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(v) * (1.f / 2.4f)),
+                                         1.055f * 255.f, -0.055f * 255.f);
+    return tmin(tmax((int)(e + 0.25f), 0), 255);
+}
+
+__device__ __forceinline__ int srgb_code(const float4 *step, float v, float &dec) {
+    const int k0 = srgb_k0(v);
+    const float4 st = step[k0];
+    const bool up = v >= st.x;
+    dec = up ? st.z : st.y;
+    return k0 + (up ? 1 : 0);
+}
+
+template <bool SRGB>
+__device__ __forceinline__ void blend_dec(const float4 *step, float *d, const float4 &src, float a) {
+    const float oma = 1.f - a;
+    const float v0 = src.x * a + d[0] * oma;
+    const float v1 = src.y * a + d[1] * oma;
+    const float v2 = src.z * a + d[2] * oma;
+    const float va = a * a + d[3] * oma;
+    if (SRGB) {
+        srgb_code(step, v0, d[0]);
+        srgb_code(step, v1, d[1]);
+        srgb_code(step, v2, d[2]);
+    } else {
+        d[0] = unorm_dec(unorm_code(v0));
+        d[1] = unorm_dec(unorm_code(v1));
+        d[2] = unorm_dec(unorm_code(v2));
+    }
+    d[3] = unorm_dec(unorm_code(va));
+}
+
+template <bool SRGB>
+__device__ __forceinline__ uint32_t pack_dec(const float4 *step, const float *d) {
+    uint32_t o = 0u;
+    float t;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) o |= (uint32_t)(SRGB ? srgb_code(step, d[ch], t) : unorm_code(d[ch])) << (8 * ch);
+    return o | ((uint32_t)unorm_code(d[3]) << 24);
+}
+
+struct CovArgs {
+    int W, H, tiles_x;
+    const uint32_t *rowbeg, *rowend;
+    const uint4 *rrec;          // per row-list entry: extent, colour, item count, item base
+    const uint32_t *items, *icov;
+    const float *dec_c;         // [256] colour decode, for the paint colours
+    const float4 *step_c;       // [256] sRGB steps {thr[k+1], dec[k], dec[k+1], 0}
+    uint8_t *image;
+    int pitch;
+};
+
+constexpr int kCovList = 128;   // per-wave list of the tile's buckets
+constexpr int kCovGroup = 8;    // buckets resolved to covering items together
+
+// Scatter the starts of a long bucket's items (more than 64) into the column slots.
+__device__ __noinline__ void scatter_big(const uint32_t *items, uint32_t ib, int cnt, int tx0, uint32_t *slot) {
+    const int lane = threadIdx.x & 63;
+    for (int ic = 0; ic < cnt; ic += 64) {
+        if (ic + lane < cnt) {
+            const uint32_t it = items[ib + ic + lane];
+            const int x0 = (int)(it & 0xFFFFu), iw = (int)(it >> 16);
+            const int end = x0 + (iw ? iw : 2);
+            if (end > tx0 && x0 < tx0 + 64) atomicMax(&slot[tmax(x0 - tx0, 0)], (uint32_t)(ic + lane + 1));
+        }
+    }
+}
+
+// One wavefront per 64x2 pixel tile (lane = pixel column, rows 2r and 2r+1).  The cell
+// row's non-empty buckets, in path order, are filtered by extent into a per-wave list
+// (paint colour decoded, item base and count).  The list is resolved kCovGroup buckets at
+// a time: each item writes its index into the LDS slot of its first column in the tile
+// (items of a bucket are disjoint and sorted by x), an inclusive max-scan over the lanes
+// then names each column's covering candidate, and the lanes blend in path order:
+// coverage alpha = paint alpha * covered samples / N (output_{8,32}.frag.glsl).
+template <int SPP, bool SRGB>
+__global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
+    __shared__ float4 s_step[SRGB ? 256 : 1];
+    __shared__ float s_dec[256];
+    __shared__ float4 l_src[4][kCovList];
+    __shared__ uint2 l_ref[4][kCovList];
+    __shared__ uint32_t slot[4][kCovGroup][64];
+    for (int i = threadIdx.x; i < 256; i += 256) {
+        if (SRGB) s_step[i] = A.step_c[i];
+        s_dec[i] = A.dec_c[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tile = blockIdx.x * 4 + w;
+    const int R = (A.H + 1) >> 1;
+    if (tile >= R * A.tiles_x) return;
+    const int r = tile / A.tiles_x;
+    const int tx0 = (tile - r * A.tiles_x) * 64;
+    const int tx1 = tmin(tx0 + 64, A.W);
+    const int x = tx0 + lane;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    float d0[4] = {1.f, 1.f, 1.f, 1.f}, d1[4] = {1.f, 1.f, 1.f, 1.f};
+
+    auto flush = [&](int nlist) {
+        wave_sync_lds();
+        for (int g0 = 0; g0 < nlist; g0 += kCovGroup) {
+            const int ng = tmin(kCovGroup, nlist - g0);
+            uint32_t it[kCovGroup], cv[kCovGroup];
+#pragma unroll
+            for (int g = 0; g < kCovGroup; ++g) {
+                slot[w][g][lane] = 0u;
+                const uint2 ref = g < ng ? l_ref[w][g0 + g] : make_uint2(0u, 0u);
+                const bool ld = lane < (int)ref.y;
+                it[g] = ld ? A.items[ref.x + lane] : 0u;
+                cv[g] = ld ? A.icov[ref.x + lane] : 0u;
+            }
+            wave_sync_lds();
+#pragma unroll
+            for (int g = 0; g < kCovGroup; ++g) {
+                if (g >= ng) break;
+                const uint2 ref = l_ref[w][g0 + g];
+                if (ref.y > 64u) {
+                    scatter_big(A.items, ref.x, (int)ref.y, tx0, slot[w][g]);
+                } else if (lane < (int)ref.y) {
+                    const int x0 = (int)(it[g] & 0xFFFFu), iw = (int)(it[g] >> 16);
+                    const int end = x0 + (iw ? iw : 2);
+                    if (end > tx0 && x0 < tx0 + 64) atomicMax(&slot[w][g][tmax(x0 - tx0, 0)], (uint32_t)(lane + 1));
+                }
+            }
+            wave_sync_lds();
+            uint32_t sv[kCovGroup];
+#pragma unroll
+            for (int g = 0; g < kCovGroup; ++g) sv[g] = slot[w][g][lane];
+#pragma unroll
+            for (int g = 0; g < kCovGroup; ++g) sv[g] = (uint32_t)dpp_scan((int)sv[g], 0, OpMax());
+#pragma unroll
+            for (int g = 0; g < kCovGroup; ++g) {
+                if (g >= ng) break;
+                const uint2 ref = l_ref[w][g0 + g];
+                const float4 src = l_src[w][g0 + g];
+                const int p = (int)sv[g] - 1;
+                uint32_t pit, pcv;
+                if (ref.y > 64u) {
+                    pit = p >= 0 ? A.items[ref.x + p] : 0u;
+                    pcv = p >= 0 ? A.icov[ref.x + p] : 0u;
+                } else {
+                    pit = (uint32_t)__shfl((int)it[g], tmax(p, 0), 64);
+                    pcv = (uint32_t)__shfl((int)cv[g], tmax(p, 0), 64);
+                }
+                const int px0 = (int)(pit & 0xFFFFu), pw = (int)(pit >> 16);
+                const int pend = px0 + (pw ? pw : 2);
+                if (p >= 0 && x < pend && x < A.W) {
+                    uint32_t n0 = SPP, n1 = SPP;
+                    if (pw == 0) {
+                        n0 = (pcv >> (6 * ((x - px0) * 2))) & 63u;
+                        n1 = (pcv >> (6 * ((x - px0) * 2 + 1))) & 63u;
+                    }
+                    blend_dec<SRGB>(s_step, d0, src, src.w * ((float)n0 / (float)SPP));
+                    blend_dec<SRGB>(s_step, d1, src, src.w * ((float)n1 / (float)SPP));
+                }
+            }
+            wave_sync_lds();
+        }
+    };
+
+    const uint32_t q0 = A.rowbeg[r], q1 = A.rowend[r];
+    int nlist = 0;
+    uint4 nxt = (q0 + lane < q1) ? A.rrec[q0 + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
+    for (uint32_t qc = q0; qc < q1; qc += 64) {
+        const uint4 rec = nxt;
+        const bool inq = qc + lane < q1;
+        if (qc + 64 < q1) nxt = (qc + 64 + lane < q1) ? A.rrec[qc + 64 + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
+        const int exlo = (int)(rec.x & 0xFFFFu), exhi = (int)(rec.x >> 16);
+        const bool ov = inq && exlo < tx1 && exhi > tx0;
+        const unsigned long long om = __ballot(ov);
+        if (!om) continue;
+        const int nov = (int)__popcll(om);
+        if (nlist + nov > kCovList) {
+            flush(nlist);
+            nlist = 0;
+        }
+        if (ov) {
+            const int sl = nlist + (int)__popcll(om & lt);
+            const uint32_t c = rec.y;
+            l_src[w][sl] = make_float4(s_dec[c & 0xFFu], s_dec[(c >> 8) & 0xFFu], s_dec[(c >> 16) & 0xFFu],
+                                       unorm_dec((int)(c >> 24)));
+            l_ref[w][sl] = make_uint2(rec.w, rec.z);
+        }
+        nlist += nov;
+    }
+    if (nlist) flush(nlist);
+
+    if (x >= A.W) return;
+    const int y0 = 2 * r;
+    *(uint32_t *)(A.image + (size_t)y0 * A.pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, d0);
+    if (y0 + 1 < A.H) *(uint32_t *)(A.image + (size_t)(y0 + 1) * A.pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, d1);
+}
+
+// Row-list records for the coverage compositor: one 16-byte load per listed bucket.
+__global__ void k_row_records(int64_t NB, const uint32_t *__restrict__ n_ptr, const uint32_t *__restrict__ rowvals,
+                              const uint32_t *__restrict__ bext, const int2 *__restrict__ binfo,
+                              const uint32_t *__restrict__ finfo, const uint32_t *__restrict__ bicnt,
+                              const uint32_t *__restrict__ boff, uint4 *__restrict__ rrec) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= NB || q >= (int64_t)*n_ptr) return;
+    const uint32_t bb = rowvals[q];
+    rrec[q] = make_uint4(bext[bb], finfo[binfo[bb].x], bicnt[bb], 2u * boff[bb]);
+}
+
+// Device requantisation of arbitrary values (colour-table self-check): codes and the
+// decoded values of the codes.
+template <bool SRGB>
+__global__ void k_encode_values(int64_t n, const float *__restrict__ v, const float4 *__restrict__ step,
+                                uint8_t *__restrict__ code, float *__restrict__ dec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float d;
+    int k;
+    if (SRGB) {
+        k = srgb_code(step, v[i], d);
+    } else {
+        k = unorm_code(v[i]);
+        d = unorm_dec(k);
+    }
+    code[i] = (uint8_t)k;
+    dec[i] = d;
+}
+
+// Exhaustive comparison of the device requantisation with the threshold tables over every
+// float bit pattern in [lo, hi): counts disagreements of code or decoded value.
+template <bool SRGB>
+__global__ void k_encode_sweep(uint32_t lo, uint32_t hi, const float4 *__restrict__ step, const float *__restrict__ thr,
+                               const float *__restrict__ dec, unsigned long long *bad) {
+    unsigned long long nbad = 0;
+    for (uint64_t b = (uint64_t)lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < hi;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const float v = __uint_as_float((uint32_t)b);
+        int ref = 0;
+#pragma unroll 1
+        for (int s = 128; s >= 1; s >>= 1)
+            if (ref + s <= 255 && v >= thr[ref + s]) ref += s;
+        float d;
+        int k;
+        if (SRGB) {
+            k = srgb_code(step, v, d);
+        } else {
+            k = unorm_code(v);
+            d = unorm_dec(k);
+        }
+        nbad += (k != ref || __float_as_uint(d) != __float_as_uint(dec[ref])) ? 1ull : 0ull;
+    }
+#pragma unroll
+    for (int dd = 32; dd >= 1; dd >>= 1) nbad += __shfl_xor(nbad, dd, 64);
+    if ((threadIdx.x & 63) == 0 && nbad) atomicAdd(bad, nbad);
 }
 
 // draw list + merged masks in the reference's order and layout (.cu:789-868, 1195-1266)
@@ -723,7 +1122,7 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
     ResolveArgs A;
     A.NB = f.NB; A.binfo = f.binfo; A.boff = f.boff; A.bcarry = f.bcarry; A.frule = sc.frule;
     A.rec_meta = f.rec_meta; A.rec_mask = f.rec_mask; A.scratch = f.scratch_keys; A.W = f.W;
-    A.items = f.items; A.imask = f.imask; A.bicnt = f.bicnt; A.bmcnt = f.bmcnt; A.bext = f.bext;
+    A.items = f.items; A.imask = f.imask; A.icov = f.icov; A.bicnt = f.bicnt; A.bmcnt = f.bmcnt; A.bext = f.bext;
     A.big_list = f.big_list; A.big_count = f.big_count; A.err = f.err_flag;
     launch_fill_u32(f.big_count, 0u, 1, s);
     const unsigned g1 = (unsigned)std::min<int64_t>(f.NB, kResolveBlocks);
@@ -779,27 +1178,67 @@ void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s) {
 }
 
 void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {
+    const int R = (f.H + 1) / 2;
+    const int tiles_x = (f.W + 63) / 64;
+    const int ntiles = R * tiles_x;
+    if (!f.ms) {
+        if (f.NB)
+            hipLaunchKernelGGL(k_row_records, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB,
+                               (const uint32_t *)(f.nz_off + f.NB), (const uint32_t *)f.nz_vals[0],
+                               (const uint32_t *)f.bext, (const int2 *)f.binfo, sc.finfo, (const uint32_t *)f.bicnt,
+                               (const uint32_t *)f.boff, f.rrec);
+        CovArgs C;
+        C.W = f.W; C.H = f.H; C.tiles_x = tiles_x;
+        C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec;
+        C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.step_c = t.step_c;
+        C.image = f.image; C.pitch = f.pitch;
+        dim3 g(blocks_for(ntiles, 4)), b(256);
+#define SL_LAUNCH(N, SR) hipLaunchKernelGGL((k_composite_cov<N, SR>), g, b, 0, s, C)
+        if (f.srgb) {
+            if (t.spp == 8) SL_LAUNCH(8, true);
+            else if (t.spp == 16) SL_LAUNCH(16, true);
+            else SL_LAUNCH(32, true);
+        } else {
+            if (t.spp == 8) SL_LAUNCH(8, false);
+            else if (t.spp == 16) SL_LAUNCH(16, false);
+            else SL_LAUNCH(32, false);
+        }
+#undef SL_LAUNCH
+        return;
+    }
     CompArgs A;
     A.W = f.W; A.H = f.H; A.tiles_x = (f.W + 63) / 64;
     A.rowbeg = f.rowbeg; A.rowend = f.rowend; A.rowvals = f.nz_vals[0];
     A.binfo = f.binfo; A.finfo = sc.finfo; A.bicnt = f.bicnt; A.boff = f.boff; A.bext = f.bext;
-    A.items = f.items; A.imask = f.imask;
+    A.items = f.items; A.imask = f.imask; A.icov = f.icov;
     A.dec_c = t.dec_c; A.thr_c = t.thr_c; A.dec_a = t.dec_a; A.thr_a = t.thr_a; A.lut_c = t.lut_c; A.lut_a = t.lut_a;
     A.image = f.image; A.pitch = f.pitch;
-    const int R = (f.H + 1) / 2;
-    const int ntiles = R * A.tiles_x;
     dim3 g(blocks_for(ntiles, 4)), b(256);
-#define SL_LAUNCH(N, MS) hipLaunchKernelGGL((k_composite<N, MS>), g, b, 0, s, A)
-    if (!f.ms) {
-        if (t.spp == 8) SL_LAUNCH(8, false);
-        else if (t.spp == 16) SL_LAUNCH(16, false);
-        else SL_LAUNCH(32, false);
-    } else {
-        if (t.spp == 8) SL_LAUNCH(8, true);
-        else if (t.spp == 16) SL_LAUNCH(16, true);
-        else SL_LAUNCH(32, true);
-    }
-#undef SL_LAUNCH
+    if (t.spp == 8) hipLaunchKernelGGL((k_composite<8, true>), g, b, 0, s, A);
+    else if (t.spp == 16) hipLaunchKernelGGL((k_composite<16, true>), g, b, 0, s, A);
+    else hipLaunchKernelGGL((k_composite<32, true>), g, b, 0, s, A);
+}
+
+void launch_encode_values(const Tables &t, int channel, const float *v, int64_t n, uint8_t *code, float *dec,
+                          hipStream_t s) {
+    if (n <= 0) return;
+    dim3 g(blocks_for(n, 256)), b(256);
+    if (channel == 0 && t.srgb)
+        hipLaunchKernelGGL(k_encode_values<true>, g, b, 0, s, n, v, t.step_c, code, dec);
+    else
+        hipLaunchKernelGGL(k_encode_values<false>, g, b, 0, s, n, v, t.step_c, code, dec);
+}
+
+void launch_encode_sweep(const Tables &t, int channel, uint32_t lo, uint32_t hi, unsigned long long *bad,
+                         hipStream_t s) {
+    if (hi <= lo) return;
+    const bool srgb = channel == 0 && t.srgb;
+    const float *thr = channel == 0 ? t.thr_c : t.thr_a;
+    const float *dec = channel == 0 ? t.dec_c : t.dec_a;
+    if (srgb)
+        hipLaunchKernelGGL(k_encode_sweep<true>, dim3(8192), dim3(256), 0, s, lo, hi, t.step_c, thr, dec, bad);
+    else
+        hipLaunchKernelGGL(k_encode_sweep<false>, dim3(8192), dim3(256), 0, s, lo, hi, t.step_c, thr, dec, bad);
 }
 
 void launch_debug_draw_list(const SceneDev &sc, const Tables &t, Frame &f, const uint32_t *moff, const uint32_t *ioff,

@@ -21,7 +21,7 @@ EXPORTS = (
     "sl_add_vg", "sl_clear", "sl_rasterize", "sl_output_device_ptr", "sl_read_output",
     "sl_element_number", "sl_enable_step_timing", "sl_step_timing", "sl_step_name",
     "sl_pixel_count", "sl_last_error", "sl_debug_enable", "sl_debug_dump",
-    "sl_scene_generate", "sl_scene_view", "sl_scene_free", "sl_version",
+    "sl_debug_requantize", "sl_debug_requantize_sweep", "sl_scene_generate", "sl_scene_view", "sl_scene_free", "sl_version",
 )
 
 
@@ -84,6 +84,8 @@ def lib():
         "sl_last_error": (ctypes.c_char_p, [vp]),
         "sl_debug_enable": (i, [vp, i]),
         "sl_debug_dump": (i, [vp, ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+        "sl_debug_requantize": (i, [vp, i, vp, ctypes.c_int64, vp, vp]),
+        "sl_debug_requantize_sweep": (i, [vp, i, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(u64)]),
         "sl_scene_generate": (i, [i, i, i, i, u64, i, i, ctypes.POINTER(vp)]),
         "sl_scene_view": (i, [vp, ctypes.POINTER(SlScene)]),
         "sl_scene_free": (i, [vp]),

@@ -125,6 +125,24 @@ class VGRasterizer:
         self._chk(self._L.sl_pixel_count(self._h, ctypes.byref(v)))
         return v.value
 
+    def requantize(self, values, channel=0):
+        """Device 8-bit requantisation of `values` (float32): (codes, decoded values).
+        channel 0 = colour (sRGB when enabled), 1 = alpha."""
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float32).reshape(-1))
+        codes = np.empty(v.size, dtype=np.uint8)
+        dec = np.empty(v.size, dtype=np.float32)
+        self._chk(self._L.sl_debug_requantize(self._h, int(channel), v.ctypes.data, v.size, codes.ctypes.data,
+                                              dec.ctypes.data))
+        return codes, dec
+
+    def requantizeSweep(self, bits_lo, bits_hi, channel=0):
+        """Disagreements of the device requantisation with the threshold tables over every
+        float bit pattern in [bits_lo, bits_hi)."""
+        out = ctypes.c_uint64()
+        self._chk(self._L.sl_debug_requantize_sweep(self._h, int(channel), int(bits_lo), int(bits_hi),
+                                                    ctypes.byref(out)))
+        return out.value
+
     def dump(self, name):
         """Parity dump of one intermediate array in the reference's layout."""
         n = ctypes.c_size_t()

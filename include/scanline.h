@@ -167,6 +167,19 @@ int sl_debug_enable(sl_handle *h, int enable);
 int sl_debug_dump(sl_handle *h, const char *name, void *host_dst, size_t capacity,
                   size_t *nbytes);
 
+/* Colour requantisation self-check of the 8-bit framebuffer emulation (the blend
+ * re-quantisation of an RGBA8 / SRGB8_ALPHA8 colour attachment, ras_base.cpp:1044-1060).
+ * channel 0 = colour (sRGB when sRGB correction is enabled, else UNORM), 1 = alpha (UNORM).
+ * sl_debug_requantize runs the device encoder on n host values: codes[i] and the decoded
+ * value of that code.  sl_debug_requantize_sweep compares the device encoder with the
+ * threshold tables on every float bit pattern in [bits_lo, bits_hi) and returns the
+ * number of disagreements.  Neither has a counterpart in the reference (its blending is
+ * the GL driver's); both exist to prove the emulation exact. */
+int sl_debug_requantize(sl_handle *h, int channel, const float *values, int64_t n,
+                        uint8_t *codes, float *decoded);
+int sl_debug_requantize_sweep(sl_handle *h, int channel, uint32_t bits_lo, uint32_t bits_hi,
+                              uint64_t *mismatches);
+
 /* ---- seeded synthetic scenes ------------------------------------------------------------
  * Host-only generators for the benchmark workloads (splitmix64, one substream per
  * path).  kind: 1 = closed chains of cubics with control points uniform in the canvas;
