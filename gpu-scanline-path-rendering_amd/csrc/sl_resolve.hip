@@ -899,7 +899,10 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int tile = blockIdx.x * 4 + w;
+    // workgroups are dealt round-robin to the 8 XCDs: give each XCD one contiguous band
+    // of tiles (the grid is a multiple of 8) so that its L2 holds that band's row lists
+    const int per_xcd = (int)gridDim.x >> 3;
+    const int tile = (((int)blockIdx.x & 7) * per_xcd + ((int)blockIdx.x >> 3)) * 4 + w;
     const int R = (A.H + 1) >> 1;
     if (tile >= R * A.tiles_x) return;
     const int r = tile / A.tiles_x;
@@ -1192,7 +1195,7 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
         C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec;
         C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.step_c = t.step_c;
         C.image = f.image; C.pitch = f.pitch;
-        dim3 g(blocks_for(ntiles, 4)), b(256);
+        dim3 g((blocks_for(ntiles, 4) + 7) & ~7u), b(256);
 #define SL_LAUNCH(N, SR) hipLaunchKernelGGL((k_composite_cov<N, SR>), g, b, 0, s, C)
         if (f.srgb) {
             if (t.spp == 8) SL_LAUNCH(8, true);

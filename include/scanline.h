@@ -91,8 +91,9 @@ int sl_destroy(sl_handle *h);
 
 /* ---- configuration (ras_base.h:155-205) ------------------------------------------ */
 
-/* setSamples (ras_base.h:199). 8 and 32 follow the reference; 16 uses the layout
- * documented in DESIGN.md (one 32-bit word per pixel column). */
+/* setSamples (ras_base.h:199). 8 and 32 follow the reference; 16, which has no
+ * reference mask layout, keeps one 32-bit word per pixel column (bit y*16 + s) and a
+ * bit-reversal sample set like the 8-sample one. */
 int sl_set_samples(sl_handle *h, int samples);
 
 /* setMultisampleOutput (ras_base.h:200): 0 = coverage-alpha compositing
