@@ -53,6 +53,17 @@ struct sl_handle {
     double stage_ms[kStages] = {0};
     bool debug = false;
 
+    // batched scenes (the internal handles of an sl_batch): per-path canvases and the
+    // per-scene tables of the compositor
+    int nscenes = 0;
+    int64_t batch_rows = 0, batch_tiles = 0, batch_image_bytes = 0;
+    DevBuf pcanvas, stile, simg, sfirst, scarry;
+    // launch shape (results do not depend on it)
+    int resolve_grid = 0, class_grid = 0, comp_group = 8, frag_block = 256;
+    // sizes of the last eager render; a replay of the same scene reuses them
+    bool sized = false;
+    uint32_t last_F = 0, last_NCH = 0, last_NB = 0;
+
     // scene (device)
     DevBuf vtx, vpath, cvpos, ctype, cpath, cw, crev, frule, finfo;
     // tables (device)
@@ -121,6 +132,23 @@ void release(DevBuf &b) {
 }
 
 template <class T> T *P_(DevBuf &b) { return (T *)b.p; }
+
+struct I4 {
+    int x, y, z, w;
+};
+
+// saveOutputToFile's pixel order (ras_base.cpp:1249-1262): rows flipped, alpha 255
+void flip_rows_alpha(uint8_t *rgba, int W, int H) {
+    const size_t row = (size_t)W * 4;
+    std::vector<uint8_t> tmp(row);
+    for (int y = 0; y < H / 2; ++y) {
+        uint8_t *a = rgba + row * y, *b = rgba + row * (H - 1 - y);
+        std::memcpy(tmp.data(), a, row);
+        std::memcpy(a, b, row);
+        std::memcpy(b, tmp.data(), row);
+    }
+    for (size_t i = 3; i < row * H; i += 4) rgba[i] = 255;
+}
 
 int upload(sl_handle *h, DevBuf &b, const void *src, size_t bytes) {
     int rc = ensure(h, b, bytes);
@@ -191,6 +219,7 @@ sl::SceneDev scene_view(sl_handle *h) {
     s.cvpos = P_<uint32_t>(h->cvpos); s.ctype = P_<uint8_t>(h->ctype); s.cpath = P_<uint32_t>(h->cpath);
     s.cw = P_<float>(h->cw); s.crev = P_<uint8_t>(h->crev);
     s.frule = P_<uint8_t>(h->frule); s.finfo = P_<uint32_t>(h->finfo);
+    s.pcanvas = h->nscenes ? P_<int4>(h->pcanvas) : nullptr;
     return s;
 }
 
@@ -208,8 +237,12 @@ uint32_t read_u32(sl_handle *h, const uint32_t *dptr, hipStream_t s, int &rc) {
     return v;
 }
 
-int render(sl_handle *h, hipStream_t s) {
+// One frame.  `replay` re-issues the work of the previous eager render of the same scene
+// without any host synchronisation (buffer sizes and counts are taken from that render),
+// so that the launch sequence can be captured into a HIP graph or timed back to back.
+int render(sl_handle *h, hipStream_t s, bool replay = false) {
     int rc;
+    if (replay && !h->sized) return fail(h, SL_E_STATE, "replay before an eager render of the scene");
     if (!h->has_scene) return fail(h, SL_E_STATE, "rasterize called before add_vg");
     if (h->W <= 0 || h->H <= 0) return fail(h, SL_E_STATE, "output size not set (resize_output_buffer)");
     if ((rc = ensure_tables(h))) return rc;
@@ -224,6 +257,16 @@ int render(sl_handle *h, hipStream_t s) {
         for (int j = 0; j < 4; ++j) f.m[4 * i + j] = h->m[4 * j + i];
     f.srgb = h->srgb; f.ms = h->ms;
     f.pitch = h->W * 4;
+    f.R = h->nscenes ? h->batch_rows : (int64_t)(h->H + 1) / 2;
+    f.nscenes = h->nscenes;
+    f.ntiles = h->batch_tiles;
+    f.stile = P_<int4>(h->stile);
+    f.simg = P_<uint64_t>(h->simg);
+    f.sfirst = P_<uint32_t>(h->sfirst);
+    f.scarry = P_<uint32_t>(h->scarry);
+    f.resolve_grid = h->resolve_grid; f.class_grid = h->class_grid;
+    f.comp_group = h->comp_group; f.frag_block = h->frag_block;
+    const size_t image_bytes = h->nscenes ? (size_t)h->batch_image_bytes : (size_t)h->H * f.pitch;
 
     size_t scan_need = sl::scan_tmp_elems_needed((size_t)std::max(C, P) + 1);
     int rc2 = SL_OK;
@@ -235,7 +278,7 @@ int render(sl_handle *h, hipStream_t s) {
         (rc = ensure(h, h->b_pcnt, (size_t)C * 20)) || (rc = ensure(h, h->b_nchain, ((size_t)C + 1) * 4)) ||
         (rc = ensure(h, h->b_choff, ((size_t)C + 1) * 4)) ||
         (rc = ensure(h, h->b_pbase, ((size_t)P + 1) * 4)) || (rc = ensure(h, h->b_small, 64)) ||
-        (rc = ensure(h, h->b_image, (size_t)h->H * f.pitch)) || (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
+        (rc = ensure(h, h->b_image, image_bytes)) || (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
         return rc;
     f.tvtx = P_<float2>(h->b_tvtx); f.pflags = P_<uint32_t>(h->b_pflags); f.prow = P_<int2>(h->b_prow);
     f.ccuts = P_<float4>(h->b_ccuts); f.cncut = P_<uint8_t>(h->b_cncut); f.cfrag = P_<uint32_t>(h->b_cfrag);
@@ -247,7 +290,7 @@ int render(sl_handle *h, hipStream_t s) {
     f.scan_tmp = P_<uint32_t>(h->b_scan_tmp);
 
     auto mark = [&](int i) {
-        if (h->timing) (void)hipEventRecord(h->ev[i], s);
+        if (h->timing && !replay) (void)hipEventRecord(h->ev[i], s);
     };
     mark(0);
     sl::launch_fill_u32(f.err_flag, 0u, 1, s);
@@ -260,23 +303,31 @@ int render(sl_handle *h, hipStream_t s) {
     sl::scan_exclusive_u32(f.nchain, f.choff, C, f.scan_tmp, s);
     if ((rc = check_launch(h, "monotone"))) return rc;
     uint32_t F = 0, NCH = 0;
-    HIPCHK(h, hipMemcpyAsync(&F, f.coff + C, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(h, hipMemcpyAsync(&NCH, f.choff + C, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(h, hipStreamSynchronize(s));
+    if (replay) {
+        F = h->last_F;
+        NCH = h->last_NCH;
+    } else {
+        h->sized = false;
+        HIPCHK(h, hipMemcpyAsync(&F, f.coff + C, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(h, hipMemcpyAsync(&NCH, f.choff + C, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(h, hipStreamSynchronize(s));
+        h->counts.path = P; h->counts.curve = C; h->counts.vertex = V;
+        h->counts.curve_fragment = F;
+        h->counts.merged_fragment = 0;
+        h->counts.span = 0;
+        h->pixel_count = 0;
+        h->NB = 0;
+    }
     f.F = F;
     f.NCH = NCH;
-    h->counts.path = P; h->counts.curve = C; h->counts.vertex = V;
-    h->counts.curve_fragment = F;
-    h->counts.merged_fragment = 0;
-    h->counts.span = 0;
-    h->pixel_count = 0;
-    h->NB = 0;
     if (F == 0) {
         // nothing to draw: the white clear of the reference's output pass
-        sl::launch_fill_u8(f.image, 255, (size_t)h->H * f.pitch, s);
+        sl::launch_fill_u8(f.image, 255, image_bytes, s);
         for (int i = 2; i <= kStages; ++i) mark(i);
         if ((rc = check_launch(h, "clear"))) return rc;
+        if (replay) return SL_OK;
         HIPCHK(h, hipStreamSynchronize(s));
+        h->last_F = 0; h->last_NCH = 0; h->last_NB = 0; h->sized = true;
         return SL_OK;
     }
     if ((rc = ensure(h, h->b_ts, (size_t)F * 8)) || (rc = ensure(h, h->b_chains, (size_t)NCH * 4)) ||
@@ -291,11 +342,11 @@ int render(sl_handle *h, hipStream_t s) {
     mark(3);
     sl::launch_path_buckets(sc, f, s);
     sl::scan_exclusive_u32(f.pnb, f.pbase, P, f.scan_tmp, s);
-    const uint32_t NB = read_u32(h, f.pbase + P, s, rc);
+    const uint32_t NB = replay ? h->last_NB : read_u32(h, f.pbase + P, s, rc);
     if (rc) return rc;
     f.NB = NB;
-    h->NB = NB;
-    const size_t R = (size_t)(h->H + 1) / 2;
+    if (!replay) h->NB = NB;
+    const size_t R = (size_t)f.R;
     const size_t rs_tiles = ((size_t)NB + 2047) / 2048;
     scan_need = sl::scan_tmp_elems_needed(std::max<size_t>({(size_t)C + 1, (size_t)P + 1, (size_t)NB + 1, 256 * rs_tiles + 1}));
     if ((rc = ensure(h, h->b_binfo, (size_t)NB * 8)) || (rc = ensure(h, h->b_bcount, ((size_t)NB + 1) * 4)) ||
@@ -333,6 +384,7 @@ int render(sl_handle *h, hipStream_t s) {
     sl::launch_frag_count(sc, tb, f, s);
     sl::scan_exclusive_u32(f.bcount, f.boff, NB, f.scan_tmp, s);
     sl::scan_exclusive_u32(f.bwsum, f.bcarry, NB, f.scan_tmp, s);
+    sl::launch_scene_carry(sc, f, s);
     sl::launch_frag_scatter(sc, tb, f, s);
     if ((rc = check_launch(h, "fragments"))) return rc;
     mark(5);
@@ -346,6 +398,7 @@ int render(sl_handle *h, hipStream_t s) {
     sl::launch_composite(sc, tb, f, s);
     if ((rc = check_launch(h, "composite"))) return rc;
     mark(8);
+    if (replay) return SL_OK;
 
     // element counts (M, S) and the error flag
     unsigned long long *red = (unsigned long long *)(P_<uint32_t>(h->b_small) + 4);
@@ -360,6 +413,7 @@ int render(sl_handle *h, hipStream_t s) {
     h->counts.span = (int64_t)(MS - M);
     if (small[1] & 1u) return fail(h, SL_E_HIP, "internal error: fragment row outside its path's bucket range");
     if (small[1] & 2u) return fail(h, SL_E_UNSUPPORTED, "a (path, row) bucket holds more than 65536 fragments");
+    h->last_F = F; h->last_NCH = NCH; h->last_NB = NB; h->sized = true;
 
     if (h->timing) {
         for (int i = 0; i < kStages; ++i) {
@@ -470,7 +524,8 @@ int sl_destroy(sl_handle *h) {
                       &h->b_binfo, &h->b_bcount, &h->b_boff, &h->b_bwsum, &h->b_bcarry, &h->b_bcursor, &h->b_rec_meta,
                       &h->b_rec_mask, &h->b_scratch, &h->b_items, &h->b_imask, &h->b_icov, &h->b_bicnt, &h->b_bmcnt, &h->b_bext,
                       &h->b_big_list, &h->b_small, &h->b_nz_flag, &h->b_nz_off, &h->b_nzk0, &h->b_nzk1, &h->b_nzv0,
-                      &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp};
+                      &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp,
+                      &h->b_rrec, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
     for (DevBuf *b : bufs) release(*b);
     if (h->events_ready)
         for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(h->ev[i]);
@@ -602,16 +657,7 @@ int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags) {
     if (!h->b_image.p) return fail(h, SL_E_STATE, "no image rendered yet");
     const size_t row = (size_t)h->W * 4;
     HIPCHK(h, hipMemcpy(host_rgba, h->b_image.p, row * h->H, hipMemcpyDeviceToHost));
-    if (flags & 1) {
-        std::vector<uint8_t> tmp(row);
-        for (int y = 0; y < h->H / 2; ++y) {
-            uint8_t *a = host_rgba + row * y, *b = host_rgba + row * (h->H - 1 - y);
-            std::memcpy(tmp.data(), a, row);
-            std::memcpy(a, b, row);
-            std::memcpy(b, tmp.data(), row);
-        }
-        for (size_t i = 3; i < row * h->H; i += 4) host_rgba[i] = 255;
-    }
+    if (flags & 1) flip_rows_alpha(host_rgba, h->W, h->H);
     return SL_OK;
 }
 
@@ -735,5 +781,312 @@ int sl_debug_dump(sl_handle *h, const char *name, void *host_dst, size_t capacit
     if (host_dst && capacity >= n && n) std::memcpy(host_dst, src, n);
     return SL_OK;
 }
+
+
+/* ---- batched scenes ------------------------------------------------------------------ */
+
+}  // extern "C"
+
+struct sl_batch {
+    int device = 0;
+    int n = 0;
+    std::vector<sl_handle *> grp;        // one internal handle per sample count present
+    std::vector<int> grp_spp;
+    std::vector<int> scene_grp, scene_w, scene_h;
+    std::vector<uint64_t> scene_off;      // byte offset of each scene's image in its group
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int shape[4] = {0, 0, 8, 256};        // resolve grid, class grid, compositor group, fragment block
+    std::string err;
+};
+
+namespace {
+
+int bfail(sl_batch *b, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    return code;
+}
+
+void drop_graph(sl_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void apply_shape(sl_batch *b) {
+    for (sl_handle *h : b->grp) {
+        h->resolve_grid = b->shape[0];
+        h->class_grid = b->shape[1];
+        h->comp_group = b->shape[2];
+        h->frag_block = b->shape[3];
+    }
+}
+
+int enqueue_all(sl_batch *b, hipStream_t s, bool replay) {
+    for (sl_handle *h : b->grp) {
+        const int rc = render(h, s, replay);
+        if (rc) return bfail(b, rc, "%s", h->err.c_str());
+    }
+    return SL_OK;
+}
+
+int capture(sl_batch *b) {
+    drop_graph(b);
+    hipError_t e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return bfail(b, SL_E_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    int rc = enqueue_all(b, b->cap_stream, true);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(b->cap_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(b, SL_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(b);
+        return bfail(b, SL_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return SL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sl_batch_create(int device, const sl_scene *scenes, const int *widths, const int *heights, const int *samples,
+                    int n_scenes, int srgb, sl_batch **out) {
+    if (!out || n_scenes <= 0 || !scenes || !widths || !heights || !samples) return SL_E_ARG;
+    *out = nullptr;
+    for (int i = 0; i < n_scenes; ++i) {
+        if (widths[i] <= 0 || heights[i] <= 0 || widths[i] > 16384 || heights[i] > 16384) return SL_E_ARG;
+        if (samples[i] != 8 && samples[i] != 16 && samples[i] != 32) return SL_E_ARG;
+    }
+    sl_batch *b = new (std::nothrow) sl_batch;
+    if (!b) return SL_E_OOM;
+    b->device = device < 0 ? 0 : device;
+    b->n = n_scenes;
+    b->scene_grp.assign(n_scenes, -1);
+    b->scene_w.assign(widths, widths + n_scenes);
+    b->scene_h.assign(heights, heights + n_scenes);
+    b->scene_off.assign(n_scenes, 0);
+    int rc = SL_OK;
+    const int spps[3] = {8, 16, 32};
+    for (int gi = 0; gi < 3 && !rc; ++gi) {
+        std::vector<int> ids;
+        for (int i = 0; i < n_scenes; ++i)
+            if (samples[i] == spps[gi]) ids.push_back(i);
+        if (ids.empty()) continue;
+        sl_handle *h = nullptr;
+        rc = sl_create(b->device, &h);
+        if (rc) break;
+        b->grp.push_back(h);
+        b->grp_spp.push_back(spps[gi]);
+        const int g = (int)b->grp.size() - 1;
+        h->spp = spps[gi];
+        h->ms = 0;
+        h->srgb = srgb ? 1 : 0;
+        // concatenate the group's scenes: paths, curves and vertices renumbered
+        uint64_t P = 0, C = 0, V = 0;
+        for (int i : ids) { P += scenes[i].n_paths; C += scenes[i].n_curves; V += scenes[i].n_vertices; }
+        if (P >= 0xFFFFFFFFull || C >= 0xFFFFFFFFull || V >= 0xFFFFFFFFull) { rc = SL_E_ARG; break; }
+        std::vector<float> vtx(2 * V), cw(C, -1.f);
+        std::vector<uint32_t> cvpos(C), cpath(C), color(P);
+        std::vector<uint8_t> ctype(C), rule(P);
+        std::vector<I4> pc(P);
+        std::vector<I4> st(ids.size());
+        std::vector<uint64_t> simg(ids.size());
+        std::vector<uint32_t> sfirst(ids.size());
+        uint64_t p0 = 0, c0 = 0, v0 = 0, row0 = 0, tile0 = 0, img0 = 0;
+        int Wmax = 1, Hmax = 1;
+        for (size_t k = 0; k < ids.size(); ++k) {
+            const sl_scene &sc = scenes[ids[k]];
+            const int W = widths[ids[k]], H = heights[ids[k]];
+            if (sc.n_paths && (!sc.vertex || !sc.curve_vertex_pos || !sc.curve_type || !sc.curve_path_id ||
+                               !sc.fill_rule || !sc.fill_color)) { rc = SL_E_ARG; break; }
+            std::memcpy(&vtx[2 * v0], sc.vertex, (size_t)sc.n_vertices * 8);
+            for (uint32_t c = 0; c < sc.n_curves; ++c) {
+                cvpos[c0 + c] = sc.curve_vertex_pos[c] + (uint32_t)v0;
+                cpath[c0 + c] = sc.curve_path_id[c] + (uint32_t)p0;
+                ctype[c0 + c] = sc.curve_type[c];
+                if (sc.curve_arc_w) cw[c0 + c] = sc.curve_arc_w[c];
+            }
+            const int rows = (H + 1) / 2, tiles = rows * ((W + 63) / 64);
+            for (uint32_t p = 0; p < sc.n_paths; ++p) {
+                color[p0 + p] = sc.fill_color[p];
+                rule[p0 + p] = sc.fill_rule[p];
+                pc[p0 + p] = {W, H, (int)row0, (int)k};
+            }
+            st[k] = {W, H, (int)tile0, (int)row0};
+            simg[k] = img0;
+            sfirst[k] = (uint32_t)p0;
+            b->scene_grp[ids[k]] = g;
+            b->scene_off[ids[k]] = img0;
+            Wmax = std::max(Wmax, W);
+            Hmax = std::max(Hmax, H);
+            p0 += sc.n_paths; c0 += sc.n_curves; v0 += sc.n_vertices;
+            row0 += rows; tile0 += tiles;
+            img0 += ((uint64_t)W * H * 4 + 255) & ~(uint64_t)255;
+        }
+        if (rc) break;
+        if (row0 >= 0x7FFFFFFFull || tile0 >= 0x7FFFFFFFull) { rc = SL_E_ARG; break; }
+        sl_scene all{(uint32_t)P, (uint32_t)C, (uint32_t)V, vtx.data(), cvpos.data(), ctype.data(), cpath.data(),
+                     cw.data(), rule.data(), color.data()};
+        h->W = Wmax;
+        h->H = Hmax;
+        if ((rc = sl_add_vg(h, &all))) { b->err = h->err; break; }
+        h->nscenes = (int)ids.size();
+        h->batch_rows = (int64_t)row0;
+        h->batch_tiles = (int64_t)tile0;
+        h->batch_image_bytes = (int64_t)img0;
+        if ((rc = upload(h, h->pcanvas, pc.data(), pc.size() * 16)) ||
+            (rc = upload(h, h->stile, st.data(), st.size() * 16)) ||
+            (rc = upload(h, h->simg, simg.data(), simg.size() * 8)) ||
+            (rc = upload(h, h->sfirst, sfirst.data(), sfirst.size() * 4)) ||
+            (rc = ensure(h, h->scarry, ids.size() * 4))) { b->err = h->err; break; }
+    }
+    if (!rc && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) rc = SL_E_HIP;
+    if (!rc) {
+        // one eager frame sizes every buffer; the graph then replays it without host syncs
+        apply_shape(b);
+        rc = enqueue_all(b, b->cap_stream, false);
+        if (!rc && hipStreamSynchronize(b->cap_stream) != hipSuccess) rc = SL_E_HIP;
+        if (!rc) rc = capture(b);
+    }
+    if (rc) {
+        sl_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return SL_OK;
+}
+
+int sl_batch_destroy(sl_batch *b) {
+    if (!b) return SL_OK;
+    drop_graph(b);
+    for (sl_handle *h : b->grp) sl_destroy(h);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    delete b;
+    return SL_OK;
+}
+
+int sl_batch_render(sl_batch *b, void *stream) {
+    if (!b || !b->exec) return SL_E_ARG;
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, SL_E_HIP, "hipSetDevice failed");
+    const hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return bfail(b, SL_E_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
+    return SL_OK;
+}
+
+int sl_batch_render_eager(sl_batch *b, void *stream) {
+    if (!b) return SL_E_ARG;
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, SL_E_HIP, "hipSetDevice failed");
+    return enqueue_all(b, (hipStream_t)stream, true);
+}
+
+int sl_batch_set_shape(sl_batch *b, const int shape[4]) {
+    if (!b || !shape) return SL_E_ARG;
+    if (shape[0] < 0 || shape[1] < 0 || (shape[2] != 4 && shape[2] != 8) ||
+        (shape[3] != 64 && shape[3] != 128 && shape[3] != 256))
+        return bfail(b, SL_E_ARG, "launch shape out of range");
+    for (int i = 0; i < 4; ++i) b->shape[i] = shape[i];
+    apply_shape(b);
+    return capture(b);
+}
+
+int sl_batch_get_shape(sl_batch *b, int shape[4]) {
+    if (!b || !shape) return SL_E_ARG;
+    for (int i = 0; i < 4; ++i) shape[i] = b->shape[i];
+    return SL_OK;
+}
+
+int sl_batch_tune(sl_batch *b, int reps, double *best_ms) {
+    if (!b) return SL_E_ARG;
+    if (reps <= 0) reps = 3;
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, SL_E_HIP, "hipSetDevice failed");
+    // candidate launch shapes: per-bucket resolve grid, size-class grid, compositor
+    // bucket group, fragment block size
+    const int rg[] = {4096, 16384, 65536};
+    const int cg[] = {1024, 4096};
+    const int gg[] = {4, 8};
+    const int fbk[] = {128, 256};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return bfail(b, SL_E_HIP, "hipEventCreate failed");
+    double best = 1e30;
+    int best_shape[4] = {b->shape[0], b->shape[1], b->shape[2], b->shape[3]};
+    int rc = SL_OK;
+    for (int a : rg)
+        for (int c : cg)
+            for (int g : gg)
+                for (int f : fbk) {
+                    if (rc) break;
+                    b->shape[0] = a; b->shape[1] = c; b->shape[2] = g; b->shape[3] = f;
+                    apply_shape(b);
+                    rc = enqueue_all(b, b->cap_stream, true);  // warm-up
+                    if (rc) break;
+                    (void)hipEventRecord(e0, b->cap_stream);
+                    for (int r = 0; r < reps && !rc; ++r) rc = enqueue_all(b, b->cap_stream, true);
+                    (void)hipEventRecord(e1, b->cap_stream);
+                    if (hipEventSynchronize(e1) != hipSuccess) rc = bfail(b, SL_E_HIP, "tuning run failed");
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (!rc && ms / reps < best) {
+                        best = ms / reps;
+                        best_shape[0] = a; best_shape[1] = c; best_shape[2] = g; best_shape[3] = f;
+                    }
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (int i = 0; i < 4; ++i) b->shape[i] = best_shape[i];
+    apply_shape(b);
+    if (rc) return rc;
+    if (best_ms) *best_ms = best;
+    return capture(b);
+}
+
+int sl_batch_element_number(sl_batch *b, sl_counts *out) {
+    if (!b || !out) return SL_E_ARG;
+    sl_counts t{};
+    for (sl_handle *h : b->grp) {
+        t.path += h->counts.path; t.curve += h->counts.curve; t.vertex += h->counts.vertex;
+        t.curve_fragment += h->counts.curve_fragment; t.merged_fragment += h->counts.merged_fragment;
+        t.span += h->counts.span;
+    }
+    *out = t;
+    return SL_OK;
+}
+
+int sl_batch_output_device_ptr(sl_batch *b, int scene, void **dev_ptr, int *pitch_bytes) {
+    if (!b || scene < 0 || scene >= b->n || !dev_ptr || !pitch_bytes) return SL_E_ARG;
+    sl_handle *h = b->grp[b->scene_grp[scene]];
+    *dev_ptr = (uint8_t *)h->b_image.p + b->scene_off[scene];
+    *pitch_bytes = 4 * b->scene_w[scene];
+    return SL_OK;
+}
+
+int sl_batch_read_output(sl_batch *b, int scene, uint8_t *host_rgba, int flags) {
+    if (!b || scene < 0 || scene >= b->n || !host_rgba) return SL_E_ARG;
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, SL_E_HIP, "hipSetDevice failed");
+    if (hipDeviceSynchronize() != hipSuccess) return bfail(b, SL_E_HIP, "render failed");
+    sl_handle *h = b->grp[b->scene_grp[scene]];
+    const int W = b->scene_w[scene], H = b->scene_h[scene];
+    const size_t bytes = (size_t)W * H * 4;
+    const hipError_t e = hipMemcpy(host_rgba, (uint8_t *)h->b_image.p + b->scene_off[scene], bytes,
+                                   hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return bfail(b, SL_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+    if (flags & 1) flip_rows_alpha(host_rgba, W, H);
+    return SL_OK;
+}
+
+const char *sl_batch_last_error(sl_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
 }  // extern "C"

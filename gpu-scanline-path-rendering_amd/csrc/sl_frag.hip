@@ -115,6 +115,7 @@ __device__ __forceinline__ Frag make_fragment(const SceneDev &sc, const uint32_t
     if (t1 == 0.f || t1 == 1.f) f1 = 1;
     const uint32_t c = (uint32_t)e0.x;
     o.path = sc.cpath[c];
+    path_canvas(sc, o.path, W, H);
     o.yx = (int32_t)0xFFFEFFFE;
     o.row = -1;
     o.pos_x = 0;
@@ -313,7 +314,8 @@ void launch_bucket_info(const SceneDev &sc, Frame &f, hipStream_t s) {
 
 void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {
     if (!f.F) return;
-    dim3 g(blocks_for(f.F, 256)), b(256);
+    const unsigned fb = f.frag_block > 0 && f.frag_block <= 256 ? (unsigned)f.frag_block : 256u;
+    dim3 g(blocks_for(f.F, fb)), b(fb);
 #define SL_LAUNCH(N)                                                                                              \
     hipLaunchKernelGGL(k_frag_count<N>, g, b, 0, s, sc, t.pixel, (const float2 *)f.tvtx, (const int2 *)f.ts, f.F, \
                        f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, f.bcount, f.bwsum, f.err_flag)
@@ -325,7 +327,8 @@ void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_
 
 void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {
     if (!f.F) return;
-    dim3 g(blocks_for(f.F, 256)), b(256);
+    const unsigned fb = f.frag_block > 0 && f.frag_block <= 256 ? (unsigned)f.frag_block : 256u;
+    dim3 g(blocks_for(f.F, fb)), b(fb);
 #define SL_LAUNCH(N)                                                                                                \
     hipLaunchKernelGGL(k_frag_scatter<N>, g, b, 0, s, sc, t.pixel, (const float2 *)f.tvtx, (const int2 *)f.ts, f.F, \
                        f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, (const uint32_t *)f.boff,         \

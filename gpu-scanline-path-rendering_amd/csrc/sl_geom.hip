@@ -114,9 +114,15 @@ __global__ void k_path_init(uint32_t P, uint32_t *pflags, int2 *prow) {
 struct Mat4 { float m[16]; };
 
 __global__ void k_transform(uint32_t V, const float2 *__restrict__ vtx, const uint32_t *__restrict__ vpath,
-                            Mat4 M, int W, int H, float2 *__restrict__ out, uint32_t *pflags) {
+                            const int4 *__restrict__ pcanvas, Mat4 M, int W, int H, float2 *__restrict__ out,
+                            uint32_t *pflags) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
+    if (pcanvas) {
+        const int4 cv = pcanvas[vpath[i]];
+        W = cv.x;
+        H = cv.y;
+    }
     float2 iv = vtx[i];
     const float ix = iv.x, iy = iv.y, iz = 0.f, iw = 1.f;
     float o[4];
@@ -249,6 +255,7 @@ __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__r
     if (c >= sc.C) return;
     Curve cu = load_curve(sc, tv, c);
     const bool vis = region_visible(pflags[sc.cpath[c]]);
+    path_canvas(sc, sc.cpath[c], W, H);
     float tq[4] = {0.f, 0.f, 0.f, 0.f};
     int n = 0;
     if (vis) {
@@ -362,6 +369,7 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
     const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
     const uint32_t cnt = side ? ny : nx;
     if (cnt == 0) return;
+    path_canvas(sc, sc.cpath[c], W, H);
     Curve cu = load_curve(sc, tv, c);
     const float4 q = ccuts[c];
     const int n = cncut[c];
@@ -404,6 +412,8 @@ __global__ void __launch_bounds__(256) k_merge(SceneDev sc, const float2 *__rest
     if (c >= sc.C) return;
     const uint32_t base = coff[c];
     if (coff[c + 1] == base) return;
+    int W = 0;
+    path_canvas(sc, sc.cpath[c], W, H);
     Curve cu = load_curve(sc, tv, c);
     const float4 q = ccuts[c];
     const int n = cncut[c];
@@ -506,8 +516,8 @@ void launch_transform(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.V) return;
     Mat4 M;
     for (int i = 0; i < 16; ++i) M.m[i] = f.m[i];
-    hipLaunchKernelGGL(k_transform, dim3(blocks_for(sc.V, 256)), dim3(256), 0, s, sc.V, sc.vtx, sc.vpath, M, f.W, f.H,
-                       f.tvtx, f.pflags);
+    hipLaunchKernelGGL(k_transform, dim3(blocks_for(sc.V, 256)), dim3(256), 0, s, sc.V, sc.vtx, sc.vpath, sc.pcanvas, M,
+                       f.W, f.H, f.tvtx, f.pflags);
 }
 
 void launch_monotone(const SceneDev &sc, Frame &f, hipStream_t s) {

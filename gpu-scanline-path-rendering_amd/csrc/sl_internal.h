@@ -20,7 +20,18 @@ struct SceneDev {
     const uint8_t *crev;     // [C] 1 = vertex order was reversed at upload
     const uint8_t *frule;    // [P]
     const uint32_t *finfo;   // [P] fill_info (RGBA8, alpha 0 -> 0)
+    const int4 *pcanvas;     // [P] batched scenes: {W, H, first cell row, scene} of the
+                             //     path's own canvas; null for a single scene
 };
+
+// Canvas of a path: the frame's, or the path's own scene's in a batch.
+__device__ __forceinline__ void path_canvas(const SceneDev &sc, uint32_t path, int &W, int &H) {
+    if (sc.pcanvas) {
+        const int4 c = sc.pcanvas[path];
+        W = c.x;
+        H = c.y;
+    }
+}
 
 struct Tables {
     int spp;                 // 8, 16 or 32
@@ -85,6 +96,17 @@ struct Frame {
     uint32_t *nz_vals[2];    // [NB] bucket ids (ping-pong)
     uint32_t *rs_hist;       // radix histogram scratch
     uint32_t *rs_hoff;
+    int64_t R;               // cell rows (of all scenes of a batch)
+    // batched scenes (nscenes > 0): per scene {W, H, first tile, first cell row}, image
+    // byte offsets, first path, and the winding-scan carry at its first bucket
+    int nscenes;
+    int64_t ntiles;
+    const int4 *stile;
+    const uint64_t *simg;
+    const uint32_t *sfirst;
+    uint32_t *scarry;
+    // launch shape (autotuned for batches; results do not depend on it)
+    int resolve_grid, class_grid, comp_group, frag_block;
     uint32_t *rowbeg;        // [R]
     uint32_t *rowend;        // [R]
     uint4 *rrec;             // [NB] row-list records {extent, colour, items, item base}
@@ -109,6 +131,7 @@ void launch_bucket_info(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
 void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
 void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
+void launch_scene_carry(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
 void launch_encode_values(const Tables &t, int channel, const float *v, int64_t n, uint8_t *code, float *dec,

@@ -132,6 +132,8 @@ struct ResolveArgs {
     const uint32_t *rec_mask;
     uint32_t *scratch;
     int W;
+    const int4 *pcanvas;     // batched scenes: per-path canvas (W, H, row base, scene)
+    const uint32_t *scarry;  // batched scenes: winding-scan carry at each scene's first bucket
     uint32_t *items;
     uint32_t *imask;
     uint32_t *icov;
@@ -209,6 +211,13 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
 
     const int rule = A.frule[A.binfo[b].x];
     uint32_t wcarry = A.bcarry[b];
+    int W = A.W;
+    if (A.pcanvas) {
+        // each scene of a batch scans its winding numbers from zero
+        const int4 cv = A.pcanvas[A.binfo[b].x];
+        W = cv.x;
+        wcarry -= A.scarry[cv.w];
+    }
     int icarry = 0, nmerged = 0, xmin = INT_MAX, xmax = INT_MIN;
     const size_t ib = 2 * (size_t)base;
 #pragma unroll
@@ -226,7 +235,7 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
         const int8_t w_at = (int8_t)(int)(wcarry + (uint32_t)(incl - swn));
         wcarry += (uint32_t)lane63(incl);
         const int x = 2 * cx - 2;
-        const bool merged = head && x >= 0 && x < A.W;
+        const bool merged = head && x >= 0 && x < W;
         uint32_t res[NW];
         if (merged) {
             Stencil<NW> st;
@@ -345,6 +354,12 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
         }
         const int rule = A.frule[A.binfo[b].x];
         uint32_t wcarry = A.bcarry[b];
+        int W = A.W;
+        if (A.pcanvas) {
+            const int4 cv = A.pcanvas[A.binfo[b].x];
+            W = cv.x;
+            wcarry -= A.scarry[cv.w];
+        }
         int icarry = 0, nmerged = 0, xmin = INT_MAX, xmax = INT_MIN;
         const size_t ib = 2 * (size_t)base;
         for (int c0 = 0; c0 < n; c0 += 64) {
@@ -360,7 +375,7 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
             const int8_t w_at = (int8_t)(int)(wcarry + (uint32_t)(incl - swn));
             wcarry += (uint32_t)lane63(incl);
             const int x = 2 * cx - 2;
-            const bool merged = head && x >= 0 && x < A.W;
+            const bool merged = head && x >= 0 && x < W;
             uint32_t res[NW];
             if (merged) {
                 Stencil<NW> st;
@@ -413,12 +428,21 @@ __global__ void k_nz_flags(int64_t NB, const uint32_t *__restrict__ bicnt, uint3
     if (b < NB) flag[b] = bicnt[b] ? 1u : 0u;
 }
 __global__ void k_nz_compact(int64_t NB, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ off,
-                             const int2 *__restrict__ binfo, uint32_t *keys, uint32_t *vals) {
+                             const int2 *__restrict__ binfo, const int4 *__restrict__ pcanvas, uint32_t *keys,
+                             uint32_t *vals) {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= NB || !flag[b]) return;
     const uint32_t o = off[b];
-    keys[o] = (uint32_t)binfo[b].y;
+    const int2 bi = binfo[b];
+    keys[o] = (uint32_t)bi.y + (pcanvas ? (uint32_t)pcanvas[bi.x].z : 0u);
     vals[o] = (uint32_t)b;
+}
+
+// Winding-scan carry at the first bucket of every scene of a batch.
+__global__ void k_scene_carry(int n, const uint32_t *__restrict__ sfirst, const uint32_t *__restrict__ pbase,
+                              const uint32_t *__restrict__ bcarry, uint32_t *scarry) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) scarry[i] = bcarry[pbase[sfirst[i]]];
 }
 
 constexpr int kRsThreads = 256, kRsItems = 8, kRsTile = kRsThreads * kRsItems;
@@ -861,10 +885,14 @@ struct CovArgs {
     const float4 *step_c;       // [256] sRGB steps {thr[k+1], dec[k], dec[k+1], 0}
     uint8_t *image;
     int pitch;
+    // batched scenes: per scene {W, H, first tile, first cell row} and image byte offset
+    int nscenes;
+    int64_t ntiles;
+    const int4 *stile;
+    const uint64_t *simg;
 };
 
 constexpr int kCovList = 128;   // per-wave list of the tile's buckets
-constexpr int kCovGroup = 8;    // buckets resolved to covering items together
 
 // Scatter the starts of a long bucket's items (more than 64) into the column slots.
 __device__ __noinline__ void scatter_big(const uint32_t *items, uint32_t ib, int cnt, int tx0, uint32_t *slot) {
@@ -886,7 +914,7 @@ __device__ __noinline__ void scatter_big(const uint32_t *items, uint32_t ib, int
 // (items of a bucket are disjoint and sorted by x), an inclusive max-scan over the lanes
 // then names each column's covering candidate, and the lanes blend in path order:
 // coverage alpha = paint alpha * covered samples / N (output_{8,32}.frag.glsl).
-template <int SPP, bool SRGB>
+template <int SPP, bool SRGB, int kCovGroup>
 __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
     __shared__ float4 s_step[SRGB ? 256 : 1];
     __shared__ float s_dec[256];
@@ -899,15 +927,32 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // workgroups are dealt round-robin to the 8 XCDs: give each XCD one contiguous band
-    // of tiles (the grid is a multiple of 8) so that its L2 holds that band's row lists
-    const int per_xcd = (int)gridDim.x >> 3;
-    const int tile = (((int)blockIdx.x & 7) * per_xcd + ((int)blockIdx.x >> 3)) * 4 + w;
-    const int R = (A.H + 1) >> 1;
-    if (tile >= R * A.tiles_x) return;
-    const int r = tile / A.tiles_x;
-    const int tx0 = (tile - r * A.tiles_x) * 64;
-    const int tx1 = tmin(tx0 + 64, A.W);
+    // consecutive workgroups take consecutive tiles: the round-robin deal of workgroups to
+    // the 8 XCDs then spreads every band of rows over all XCDs, which balances the uneven
+    // per-row work better than giving each XCD a contiguous band (measured slower)
+    int tile = (int)blockIdx.x * 4 + w;
+    if (tile >= A.ntiles) return;
+    int W = A.W, H = A.H, tiles_x = A.tiles_x, row0 = 0, pitch = A.pitch;
+    uint8_t *image = A.image;
+    if (A.nscenes) {
+        // the scene whose tile range holds this tile: last one with first tile <= tile
+        int lo = 0, hi = A.nscenes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (A.stile[mid].z <= tile) lo = mid; else hi = mid - 1;
+        }
+        const int4 st = A.stile[lo];
+        W = st.x;
+        H = st.y;
+        tiles_x = (W + 63) >> 6;
+        tile -= st.z;
+        row0 = st.w;
+        pitch = 4 * W;
+        image = A.image + A.simg[lo];
+    }
+    const int r = tile / tiles_x;
+    const int tx0 = (tile - r * tiles_x) * 64;
+    const int tx1 = tmin(tx0 + 64, W);
     const int x = tx0 + lane;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     float d0[4] = {1.f, 1.f, 1.f, 1.f}, d1[4] = {1.f, 1.f, 1.f, 1.f};
@@ -960,7 +1005,7 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
                 }
                 const int px0 = (int)(pit & 0xFFFFu), pw = (int)(pit >> 16);
                 const int pend = px0 + (pw ? pw : 2);
-                if (p >= 0 && x < pend && x < A.W) {
+                if (p >= 0 && x < pend && x < W) {
                     uint32_t n0 = SPP, n1 = SPP;
                     if (pw == 0) {
                         n0 = (pcv >> (6 * ((x - px0) * 2))) & 63u;
@@ -974,7 +1019,7 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
         }
     };
 
-    const uint32_t q0 = A.rowbeg[r], q1 = A.rowend[r];
+    const uint32_t q0 = A.rowbeg[row0 + r], q1 = A.rowend[row0 + r];
     int nlist = 0;
     uint4 nxt = (q0 + lane < q1) ? A.rrec[q0 + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
     for (uint32_t qc = q0; qc < q1; qc += 64) {
@@ -1001,10 +1046,10 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
     }
     if (nlist) flush(nlist);
 
-    if (x >= A.W) return;
+    if (x >= W) return;
     const int y0 = 2 * r;
-    *(uint32_t *)(A.image + (size_t)y0 * A.pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, d0);
-    if (y0 + 1 < A.H) *(uint32_t *)(A.image + (size_t)(y0 + 1) * A.pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, d1);
+    *(uint32_t *)(image + (size_t)y0 * pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, d0);
+    if (y0 + 1 < H) *(uint32_t *)(image + (size_t)(y0 + 1) * pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, d1);
 }
 
 // Row-list records for the coverage compositor: one 16-byte load per listed bucket.
@@ -1125,17 +1170,19 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
     ResolveArgs A;
     A.NB = f.NB; A.binfo = f.binfo; A.boff = f.boff; A.bcarry = f.bcarry; A.frule = sc.frule;
     A.rec_meta = f.rec_meta; A.rec_mask = f.rec_mask; A.scratch = f.scratch_keys; A.W = f.W;
+    A.pcanvas = sc.pcanvas; A.scarry = f.scarry;
     A.items = f.items; A.imask = f.imask; A.icov = f.icov; A.bicnt = f.bicnt; A.bmcnt = f.bmcnt; A.bext = f.bext;
     A.big_list = f.big_list; A.big_count = f.big_count; A.err = f.err_flag;
     launch_fill_u32(f.big_count, 0u, 1, s);
-    const unsigned g1 = (unsigned)std::min<int64_t>(f.NB, kResolveBlocks);
+    const unsigned g1 = (unsigned)std::min<int64_t>(f.NB, f.resolve_grid > 0 ? f.resolve_grid : kResolveBlocks);
+    const unsigned gc = f.class_grid > 0 ? (unsigned)f.class_grid : 4096u;
     dim3 b64(64);
 #define SL_LAUNCH(N)                                                                  \
     do {                                                                              \
         hipLaunchKernelGGL(k_resolve_first<N>, dim3(g1), b64, 0, s, A);               \
-        hipLaunchKernelGGL((k_resolve_class<N, 2>), dim3(4096), b64, 0, s, A);        \
-        hipLaunchKernelGGL((k_resolve_class<N, 4>), dim3(4096), b64, 0, s, A);        \
-        hipLaunchKernelGGL((k_resolve_class<N, 8>), dim3(4096), b64, 0, s, A);        \
+        hipLaunchKernelGGL((k_resolve_class<N, 2>), dim3(gc), b64, 0, s, A);          \
+        hipLaunchKernelGGL((k_resolve_class<N, 4>), dim3(gc), b64, 0, s, A);          \
+        hipLaunchKernelGGL((k_resolve_class<N, 8>), dim3(gc), b64, 0, s, A);          \
         hipLaunchKernelGGL(k_resolve_huge<N>, dim3(1024), b64, 0, s, A);              \
     } while (0)
     if (t.spp == 8) SL_LAUNCH(8);
@@ -1144,20 +1191,26 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
 #undef SL_LAUNCH
 }
 
-void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s) {
+void launch_scene_carry(const SceneDev &sc, Frame &f, hipStream_t s) {
     (void)sc;
-    const int R = (f.H + 1) / 2;
+    if (!f.nscenes) return;
+    hipLaunchKernelGGL(k_scene_carry, dim3(blocks_for(f.nscenes, 256)), dim3(256), 0, s, f.nscenes, f.sfirst,
+                       (const uint32_t *)f.pbase, (const uint32_t *)f.bcarry, f.scarry);
+}
+
+void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s) {
+    const int64_t R = f.R;
     launch_fill_u32(f.rowbeg, 0u, (size_t)R, s);
     launch_fill_u32(f.rowend, 0u, (size_t)R, s);
     if (!f.NB) return;
     hipLaunchKernelGGL(k_nz_flags, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB, (const uint32_t *)f.bicnt, f.nz_flag);
     scan_exclusive_u32(f.nz_flag, f.nz_off, (size_t)f.NB, f.scan_tmp, s);
     hipLaunchKernelGGL(k_nz_compact, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB, (const uint32_t *)f.nz_flag,
-                       (const uint32_t *)f.nz_off, (const int2 *)f.binfo, f.nz_keys[0], f.nz_vals[0]);
+                       (const uint32_t *)f.nz_off, (const int2 *)f.binfo, sc.pcanvas, f.nz_keys[0], f.nz_vals[0]);
     const uint32_t *n_ptr = f.nz_off + f.NB;
     const uint32_t ntiles = (uint32_t)((f.NB + kRsTile - 1) / kRsTile);
     int bits = 0;
-    while ((1 << bits) < R) ++bits;
+    while ((int64_t)1 << bits < R) ++bits;
     const int passes = bits <= 8 ? 1 : (bits + 7) / 8;
     int cur = 0;
     for (int pss = 0; pss < passes; ++pss) {
@@ -1183,7 +1236,7 @@ void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s) {
 void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {
     const int R = (f.H + 1) / 2;
     const int tiles_x = (f.W + 63) / 64;
-    const int ntiles = R * tiles_x;
+    const int64_t ntiles = f.nscenes ? f.ntiles : (int64_t)R * tiles_x;
     if (!f.ms) {
         if (f.NB)
             hipLaunchKernelGGL(k_row_records, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB,
@@ -1195,8 +1248,13 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
         C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec;
         C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.step_c = t.step_c;
         C.image = f.image; C.pitch = f.pitch;
-        dim3 g((blocks_for(ntiles, 4) + 7) & ~7u), b(256);
-#define SL_LAUNCH(N, SR) hipLaunchKernelGGL((k_composite_cov<N, SR>), g, b, 0, s, C)
+        C.nscenes = f.nscenes; C.ntiles = ntiles; C.stile = f.stile; C.simg = f.simg;
+        dim3 g(blocks_for(ntiles, 4)), b(256);
+#define SL_LAUNCH(N, SR)                                                                   \
+    do {                                                                                   \
+        if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4>), g, b, 0, s, C); \
+        else hipLaunchKernelGGL((k_composite_cov<N, SR, 8>), g, b, 0, s, C);               \
+    } while (0)
         if (f.srgb) {
             if (t.spp == 8) SL_LAUNCH(8, true);
             else if (t.spp == 16) SL_LAUNCH(16, true);

@@ -1,11 +1,12 @@
 """MI355X comb-scanline path renderer: Python side of the C ABI (include/scanline.h)."""
 from ._abi import (CURVE_CUBIC, CURVE_LINEAR, CURVE_QUADRATIC, CURVE_RATIONAL, FILL_EVEN_ODD,
                    FILL_NONZERO, ScanlineError, lib)
+from .batch import BatchRasterizer, batch_workload
 from .rasterizer import VGRasterizer, createRasterizer
 from .scene import VGContainer, SceneArrays, config_scene, generate, rgba
 
 __all__ = [
     "CURVE_CUBIC", "CURVE_LINEAR", "CURVE_QUADRATIC", "CURVE_RATIONAL", "FILL_EVEN_ODD",
     "FILL_NONZERO", "ScanlineError", "lib", "VGRasterizer", "createRasterizer", "VGContainer",
-    "SceneArrays", "config_scene", "generate", "rgba",
+    "SceneArrays", "config_scene", "generate", "rgba", "BatchRasterizer", "batch_workload",
 ]

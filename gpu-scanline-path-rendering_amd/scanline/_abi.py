@@ -21,7 +21,10 @@ EXPORTS = (
     "sl_add_vg", "sl_clear", "sl_rasterize", "sl_output_device_ptr", "sl_read_output",
     "sl_element_number", "sl_enable_step_timing", "sl_step_timing", "sl_step_name",
     "sl_pixel_count", "sl_last_error", "sl_debug_enable", "sl_debug_dump",
-    "sl_debug_requantize", "sl_debug_requantize_sweep", "sl_scene_generate", "sl_scene_view", "sl_scene_free", "sl_version",
+    "sl_debug_requantize", "sl_debug_requantize_sweep", "sl_batch_create", "sl_batch_destroy",
+    "sl_batch_render", "sl_batch_render_eager", "sl_batch_tune", "sl_batch_set_shape", "sl_batch_get_shape",
+    "sl_batch_element_number", "sl_batch_output_device_ptr", "sl_batch_read_output", "sl_batch_last_error",
+    "sl_scene_generate", "sl_scene_view", "sl_scene_free", "sl_version",
 )
 
 
@@ -86,6 +89,17 @@ def lib():
         "sl_debug_dump": (i, [vp, ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
         "sl_debug_requantize": (i, [vp, i, vp, ctypes.c_int64, vp, vp]),
         "sl_debug_requantize_sweep": (i, [vp, i, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(u64)]),
+        "sl_batch_create": (i, [i, vp, vp, vp, vp, i, i, ctypes.POINTER(vp)]),
+        "sl_batch_destroy": (i, [vp]),
+        "sl_batch_render": (i, [vp, vp]),
+        "sl_batch_render_eager": (i, [vp, vp]),
+        "sl_batch_tune": (i, [vp, i, ctypes.POINTER(ctypes.c_double)]),
+        "sl_batch_set_shape": (i, [vp, vp]),
+        "sl_batch_get_shape": (i, [vp, vp]),
+        "sl_batch_element_number": (i, [vp, ctypes.POINTER(SlCounts)]),
+        "sl_batch_output_device_ptr": (i, [vp, i, ctypes.POINTER(vp), ctypes.POINTER(i)]),
+        "sl_batch_read_output": (i, [vp, i, vp, i]),
+        "sl_batch_last_error": (ctypes.c_char_p, [vp]),
         "sl_scene_generate": (i, [i, i, i, i, u64, i, i, ctypes.POINTER(vp)]),
         "sl_scene_view": (i, [vp, ctypes.POINTER(SlScene)]),
         "sl_scene_free": (i, [vp]),

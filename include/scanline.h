@@ -181,6 +181,31 @@ int sl_debug_requantize(sl_handle *h, int channel, const float *values, int64_t 
 int sl_debug_requantize_sweep(sl_handle *h, int channel, uint32_t bits_lo, uint32_t bits_hi,
                               uint64_t *mismatches);
 
+/* ---- batched scenes -------------------------------------------------------------------
+ * Many independent scenes of uneven size rendered by one launch sequence over all of
+ * them, captured once into a HIP graph and replayed.  No reference counterpart: the
+ * reference renders one scene per rasterize() call (ras_base.cpp:1131-1195); a batch
+ * gives each scene exactly the image a separate handle would (coverage output).
+ * sl_batch_create uploads and orients every scene (as sl_add_vg), renders one eager
+ * frame to size the buffers, and captures the graph.  sl_batch_render replays it on
+ * `stream`; sl_batch_render_eager issues the same kernels without the graph.
+ * sl_batch_tune times candidate launch shapes {per-bucket resolve grid, size-class
+ * grid, compositor bucket group, fragment block size} on this batch, keeps the fastest
+ * and recaptures.  Images are read per scene. */
+typedef struct sl_batch sl_batch;
+int sl_batch_create(int device, const sl_scene *scenes, const int *widths, const int *heights,
+                    const int *samples, int n_scenes, int srgb, sl_batch **out);
+int sl_batch_destroy(sl_batch *b);
+int sl_batch_render(sl_batch *b, void *stream);
+int sl_batch_render_eager(sl_batch *b, void *stream);
+int sl_batch_tune(sl_batch *b, int reps, double *best_ms);
+int sl_batch_set_shape(sl_batch *b, const int shape[4]);
+int sl_batch_get_shape(sl_batch *b, int shape[4]);
+int sl_batch_element_number(sl_batch *b, sl_counts *out);
+int sl_batch_output_device_ptr(sl_batch *b, int scene, void **dev_ptr, int *pitch_bytes);
+int sl_batch_read_output(sl_batch *b, int scene, uint8_t *host_rgba, int flags);
+const char *sl_batch_last_error(sl_batch *b);
+
 /* ---- seeded synthetic scenes ------------------------------------------------------------
  * Host-only generators for the benchmark workloads (splitmix64, one substream per
  * path).  kind: 1 = closed chains of cubics with control points uniform in the canvas;
