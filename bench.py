@@ -151,6 +151,88 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+def bench_diff(args):
+    """Differentiable mode: 256x256, 64 closed chains of 6 cubics, 16 sample rows per
+    pixel; forward and adjoint times (HIP events on the current stream), then 500 Adam
+    steps towards a target rendered from the scene with control points + N(0, 2 px^2)
+    and colours + N(0, 0.05^2)."""
+    import numpy as np
+    import torch
+
+    import scanline
+    from scanline import diff
+
+    scene, W, H, spp = scanline.config_scene(5)
+    r = diff.DiffRenderer(scene, W, H, samples=spp)
+    rng = np.random.default_rng(5)
+    uniq, index = diff.tie_vertices(scene)
+    idx = torch.tensor(index, device="cuda")
+    ut = torch.tensor(uniq + rng.normal(0.0, np.sqrt(2.0), uniq.shape).astype(np.float32), device="cuda")
+    ct = (r.colors0 + torch.tensor(rng.normal(0, 0.05, (scene.n_paths, 4)).astype(np.float32), device="cuda")).clamp(0, 1)
+    tgt = r.forward_raw(ut[idx].contiguous(), ct.contiguous(), keep=False)
+    pts = torch.tensor(uniq, device="cuda")[idx].contiguous()
+    g = torch.ones((H, W, 4), device="cuda")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    for _ in range(args.warmup):
+        r.forward_raw(pts, r.colors0)
+        r.backward_raw(g)
+    fwd = bwd = 0.0
+    for _ in range(args.steps):
+        ev[0].record()
+        r.forward_raw(pts, r.colors0)
+        ev[1].record()
+        r.backward_raw(g)
+        ev[2].record()
+        torch.cuda.synchronize()
+        fwd += ev[0].elapsed_time(ev[1])
+        bwd += ev[1].elapsed_time(ev[2])
+    fwd /= args.steps
+    bwd /= args.steps
+    u = torch.tensor(uniq, device="cuda", requires_grad=True)
+    col = r.colors0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([{"params": [u], "lr": 0.05}, {"params": [col], "lr": 0.005}])
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(500):
+        opt.zero_grad()
+        img = r.render(u[idx], col)
+        loss = ((img - tgt) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            col.clamp_(0, 1)
+        losses.append(float(loss.detach()))
+    adam_s = time.perf_counter() - t0
+    err0 = float((torch.tensor(uniq, device="cuda") - ut).abs().mean())
+    err1 = float((u.detach() - ut).abs().mean())
+    out = {
+        "metric": "differentiable mode: backward / forward time (256^2, 64 cubic chains, 16 sample rows)",
+        "value": round(bwd / fwd, 3),
+        "unit": "bwd/fwd",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fwd + bwd, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded cubic chains; target from perturbed points and colours)",
+        "config": {"workload": "differentiable: 256x256, 64 closed chains of 6 cubics, 16 sample rows"},
+        "forward_ms": round(fwd, 4),
+        "backward_ms": round(bwd, 4),
+        "crossings": r.crossings(),
+        "adam_500": {"loss_first": round(losses[0], 3), "loss_last": round(losses[-1], 3),
+                     "mean_point_error_px": [round(err0, 3), round(err1, 3)],
+                     "colour_error": [round(float((r.colors0 - ct).abs().mean()), 4),
+                                      round(float((col.detach() - ct).abs().mean()), 4)],
+                     "seconds": round(adam_s, 2)},
+    }
+    r.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +246,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
-    ap.add_argument("--mode", choices=["plain", "batch"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "batch", "diff"], default="plain")
     ap.add_argument("--batch-scenes", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -173,6 +255,9 @@ def main():
         sys.exit(2)
     if args.mode == "batch":
         bench_batch(args)
+        return
+    if args.mode == "diff":
+        bench_diff(args)
         return
 
     import numpy as np

@@ -24,7 +24,8 @@ EXPORTS = (
     "sl_debug_requantize", "sl_debug_requantize_sweep", "sl_batch_create", "sl_batch_destroy",
     "sl_batch_render", "sl_batch_render_eager", "sl_batch_tune", "sl_batch_set_shape", "sl_batch_get_shape",
     "sl_batch_element_number", "sl_batch_output_device_ptr", "sl_batch_read_output", "sl_batch_last_error",
-    "sl_scene_generate", "sl_scene_view", "sl_scene_free", "sl_version",
+    "sl_diff_create", "sl_diff_destroy", "sl_diff_forward", "sl_diff_backward", "sl_diff_crossings",
+    "sl_diff_last_error", "sl_scene_generate", "sl_scene_view", "sl_scene_free", "sl_version",
 )
 
 
@@ -100,6 +101,13 @@ def lib():
         "sl_batch_output_device_ptr": (i, [vp, i, ctypes.POINTER(vp), ctypes.POINTER(i)]),
         "sl_batch_read_output": (i, [vp, i, vp, i]),
         "sl_batch_last_error": (ctypes.c_char_p, [vp]),
+        "sl_diff_create": (i, [i, i, i, i, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, vp,
+                               ctypes.POINTER(vp)]),
+        "sl_diff_destroy": (i, [vp]),
+        "sl_diff_forward": (i, [vp, vp, vp, vp, i, vp]),
+        "sl_diff_backward": (i, [vp, vp, vp, vp, vp]),
+        "sl_diff_crossings": (i, [vp, ctypes.POINTER(ctypes.c_int64)]),
+        "sl_diff_last_error": (ctypes.c_char_p, [vp]),
         "sl_scene_generate": (i, [i, i, i, i, u64, i, i, ctypes.POINTER(vp)]),
         "sl_scene_view": (i, [vp, ctypes.POINTER(SlScene)]),
         "sl_scene_free": (i, [vp]),

@@ -206,6 +206,31 @@ int sl_batch_output_device_ptr(sl_batch *b, int scene, void **dev_ptr, int *pitc
 int sl_batch_read_output(sl_batch *b, int scene, uint8_t *host_rgba, int flags);
 const char *sl_batch_last_error(sl_batch *b);
 
+/* ---- differentiable mode -----------------------------------------------------------------
+ * No reference counterpart.  Coverage is a box filter of `sample_rows` rows per pixel,
+ * each integrated exactly along x (continuous in the control points); compositing is the
+ * renderer's src-over onto white in path order, unquantised, into a float RGBA image.
+ * Scenes of line / quadratic / cubic segments (closed contours), vertices stored per
+ * curve as in sl_scene.  `points` (float [V][2]), `colors` (float [P][4], RGBA in [0,1]),
+ * `image` (float [H][W][4]) and all gradients are DEVICE pointers; work runs on `stream`.
+ * sl_diff_forward keeps what the adjoint needs when `keep_for_backward` is set (the
+ * points and colours must stay unchanged until the backward).  sl_diff_backward writes
+ * d loss / d points and d loss / d colors given d loss / d image: the compositing chain
+ * reversed per pixel, and boundary terms at every sample-row crossing with a fill change.
+ * One pending backward per handle. */
+typedef struct sl_diff sl_diff;
+int sl_diff_create(int device, int width, int height, int sample_rows, uint32_t n_paths,
+                   uint32_t n_curves, uint32_t n_vertices, const uint32_t *curve_vertex_pos,
+                   const uint8_t *curve_type, const uint32_t *curve_path_id,
+                   const uint8_t *fill_rule, sl_diff **out);
+int sl_diff_destroy(sl_diff *d);
+int sl_diff_forward(sl_diff *d, const float *points, const float *colors, float *image,
+                    int keep_for_backward, void *stream);
+int sl_diff_backward(sl_diff *d, const float *grad_image, float *grad_points,
+                     float *grad_colors, void *stream);
+int sl_diff_crossings(sl_diff *d, int64_t *n);
+const char *sl_diff_last_error(sl_diff *d);
+
 /* ---- seeded synthetic scenes ------------------------------------------------------------
  * Host-only generators for the benchmark workloads (splitmix64, one substream per
  * path).  kind: 1 = closed chains of cubics with control points uniform in the canvas;

@@ -58,6 +58,9 @@ def lib(omp=False):
     L.oracle_curve_eval.restype = ctypes.c_float
     L.oracle_curve_eval.argtypes = [i, ctypes.c_float, vp, ctypes.c_float]
     L.oracle_color_tables.argtypes = [i, vp, vp]
+    if hasattr(L, "oracle_diff_run"):
+        L.oracle_diff_run.argtypes = [i, i, i, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                      ctypes.POINTER(ctypes.c_int64)]
     _libs[path] = L
     return L
 
@@ -107,3 +110,35 @@ def run(scene, width, height, samples, srgb=True, ms=False, transform=None, keep
     if not ptr:
         raise ValueError("oracle_run rejected the arguments")
     return OracleResult(L, ptr)
+
+
+def diff_run(scene, width, height, points=None, colors=None, grad_image=None, samples=16, double=True):
+    """The differentiable model on the CPU: image [H, W, 4] and, with grad_image, the
+    gradients of sum(grad_image * image) w.r.t. the vertices [V, 2] and colours [P, 4]
+    (float64 arrays for the double model, float32 for the float model).
+
+    points: float32 [V, 2] (default: the scene's vertices); colors: float32 [P, 4] in
+    [0, 1] (default: the scene's RGBA8 paint / 255)."""
+    L = lib()
+    P, C, V = scene.n_paths, scene.n_curves, scene.n_vertices
+    pts = np.ascontiguousarray(scene.vertex if points is None else points, dtype=np.float32).reshape(V, 2)
+    if colors is None:
+        c = scene.fill_color
+        colors = np.stack([(c >> (8 * k)) & 0xFF for k in range(4)], axis=1).astype(np.float32) / 255.0
+    col = np.ascontiguousarray(colors, dtype=np.float32).reshape(P, 4)
+    ft = np.float64 if double else np.float32
+    img = np.zeros((height, width, 4), dtype=ft)
+    gp = np.zeros((V, 2), dtype=ft)
+    gc = np.zeros((P, 4), dtype=ft)
+    g = None
+    if grad_image is not None:
+        g = np.ascontiguousarray(grad_image, dtype=ft).reshape(height, width, 4)
+    n = ctypes.c_int64()
+    rc = L.oracle_diff_run(1 if double else 0, int(width), int(height), int(samples), P, C, V, pts.ctypes.data,
+                           scene.curve_vertex_pos.ctypes.data, scene.curve_type.ctypes.data,
+                           scene.curve_path_id.ctypes.data, scene.fill_rule.ctypes.data, col.ctypes.data,
+                           g.ctypes.data if g is not None else None, img.ctypes.data, gp.ctypes.data,
+                           gc.ctypes.data, ctypes.byref(n))
+    if rc:
+        raise ValueError("oracle_diff_run: unsupported scene (rational segments?)")
+    return img, gp, gc, n.value

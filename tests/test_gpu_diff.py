@@ -1,0 +1,164 @@
+"""GPU: the differentiable mode against the CPU oracle and finite differences.
+
+Scene: the differentiable workload (256x256, 64 closed chains of 6 cubics, 16 sample
+rows per pixel).  The forward image is compared with the oracle's float model, the
+adjoint's gradients with the oracle's (float and double) gradients, and with central
+finite differences of the GPU forward over tied control points (a shared chain end moves
+as one point, so contours stay closed).
+
+Finite-difference tolerances.  The box-filtered, row-sampled coverage is continuous but
+only piecewise smooth in the control points: it has kinks where a crossing passes a pixel
+edge and square-root events where a y-extremum passes a sample row (the number of
+crossings changes).  Central differences with step h carry O(h) errors from those, and the
+float forward adds rounding noise of about 1e-5 px per crossing.  So the GPU check at the
+nominal h = 1e-2 px uses 2e-2 (parameters whose perturbation changes the crossing count
+are excluded); the tight checks are the adjoint against the oracle's gradients (1e-4
+against the float model, 1e-3 against the double model, whose own central differences at
+h = 1e-5 px agree to 1e-4 in tests/test_diff_oracle.py).  Colour gradients are smooth but
+see the same rounding noise through the forward: 2e-3 at h = 1e-2.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle
+import scanline
+from scanline import diff
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    scene, W, H, spp = scanline.config_scene(5)
+    r = diff.DiffRenderer(scene, W, H, samples=spp)
+    rng = np.random.default_rng(5)
+    uniq, index = diff.tie_vertices(scene)
+    # target: control points + N(0, 2 px^2), colours + N(0, 0.05^2) clamped
+    ut = uniq + rng.normal(0.0, np.sqrt(2.0), uniq.shape).astype(np.float32)
+    ct = np.clip(r.colors0.cpu().numpy() + rng.normal(0, 0.05, (scene.n_paths, 4)).astype(np.float32), 0, 1)
+    tgt_img, _, _, _ = pyoracle.diff_run(scene, W, H, points=ut[index], colors=ct, double=False)
+    yield scene, W, H, r, uniq, index, torch.tensor(tgt_img, device="cuda"), ut, ct
+    r.close()
+
+
+def loss_of(img, tgt):
+    return float(((img.double() - tgt.double()) ** 2).sum())
+
+
+def test_forward_matches_oracle(setup):
+    scene, W, H, r, uniq, index, tgt, _, _ = setup
+    print("diff forward", W, H, scene.n_paths, flush=True)
+    img = r.forward_raw(r.points0, r.colors0).cpu().numpy()
+    ref, _, _, n = pyoracle.diff_run(scene, W, H, double=False)
+    assert r.crossings() == n
+    np.testing.assert_allclose(img, ref, rtol=0, atol=1e-6)
+    refd, _, _, _ = pyoracle.diff_run(scene, W, H, double=True)
+    assert np.abs(img - refd).max() < 1e-3
+
+
+def test_gradients_match_oracle(setup):
+    scene, W, H, r, uniq, index, tgt, _, _ = setup
+    pts = r.points0.clone().requires_grad_(True)
+    col = r.colors0.clone().requires_grad_(True)
+    img = r.render(pts, col)
+    loss = ((img - tgt) ** 2).sum()
+    loss.backward()
+    g = (2 * (img - tgt)).detach().cpu().numpy()
+    _, gp32, gc32, _ = pyoracle.diff_run(scene, W, H, grad_image=g, double=False)
+    g = g.astype(np.float64)
+    _, gp64, gc64, _ = pyoracle.diff_run(scene, W, H, grad_image=g, double=True)
+    gp, gc = pts.grad.cpu().numpy(), col.grad.cpu().numpy()
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("points rel err vs float oracle %.2e, vs double %.2e" % (rel(gp, gp32), rel(gp, gp64)))
+    print("colours rel err vs float oracle %.2e, vs double %.2e" % (rel(gc, gc32), rel(gc, gc64)))
+    assert rel(gp, gp32) < 1e-4 and rel(gc, gc32) < 1e-4
+    assert rel(gp, gp64) < 1e-3 and rel(gc, gc64) < 1e-3
+
+
+def _fd_points(r, uniq, index, tgt, h, sel, rng):
+    idx_t = torch.tensor(index, device="cuda")
+    base = torch.tensor(uniq, device="cuda")
+    pts = base[idx_t].clone().requires_grad_(True)
+    img = r.render(pts, r.colors0)
+    ((img - tgt) ** 2).sum().backward()
+    # gradient of the unique points = sum over their slots
+    gu = torch.zeros_like(base).index_add_(0, idx_t, pts.grad).cpu().numpy()
+    n0 = r.crossings()
+    rows = []
+    for u in sel:
+        for d in range(2):
+            up, um = base.clone(), base.clone()
+            up[u, d] += h
+            um[u, d] -= h
+            step = float(up[u, d]) - float(um[u, d])
+            lp = loss_of(r.forward_raw(up[idx_t], r.colors0, keep=False), tgt)
+            ev = r.crossings() != n0
+            lm = loss_of(r.forward_raw(um[idx_t], r.colors0, keep=False), tgt)
+            ev = ev or r.crossings() != n0
+            rows.append(((lp - lm) / step, float(gu[u, d]), ev))
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("h,tol", [(1e-2, 2e-2)])
+def test_point_gradients_match_central_differences(setup, h, tol):
+    scene, W, H, r, uniq, index, tgt, _, _ = setup
+    rng = np.random.default_rng(17)
+    sel = rng.choice(len(uniq), 60, replace=False)
+    e = _fd_points(r, uniq, index, tgt, h, sel, rng)
+    ev = e[:, 2] > 0
+    ok = ~ev
+    rel_all = np.linalg.norm(e[:, 0] - e[:, 1]) / np.linalg.norm(e[:, 0])
+    rel_ok = np.linalg.norm(e[ok, 0] - e[ok, 1]) / np.linalg.norm(e[ok, 0])
+    print("h=%g: %d params, %d with crossing-count events; rel err all %.2e, event-free %.2e"
+          % (h, len(e), ev.sum(), rel_all, rel_ok))
+    assert rel_ok <= tol
+
+
+def test_colour_gradients_match_central_differences(setup):
+    scene, W, H, r, uniq, index, tgt, _, _ = setup
+    col = r.colors0.clone().requires_grad_(True)
+    img = r.render(r.points0, col)
+    ((img - tgt) ** 2).sum().backward()
+    g = col.grad.cpu().numpy()
+    rows = []
+    h = 1e-2
+    for p in range(0, scene.n_paths, 5):
+        for ch in range(4):
+            cp, cm = r.colors0.clone(), r.colors0.clone()
+            cp[p, ch] += h
+            cm[p, ch] -= h
+            step = float(cp[p, ch]) - float(cm[p, ch])
+            fd = (loss_of(r.forward_raw(r.points0, cp, keep=False), tgt)
+                  - loss_of(r.forward_raw(r.points0, cm, keep=False), tgt)) / step
+            rows.append((fd, g[p, ch]))
+    e = np.array(rows)
+    rel = np.linalg.norm(e[:, 0] - e[:, 1]) / np.linalg.norm(e[:, 0])
+    print("colour FD rel err %.2e" % rel)
+    assert rel <= 2e-3
+
+
+def test_adam_recovers_the_target(setup):
+    scene, W, H, r, uniq, index, tgt, ut, ct = setup
+    idx_t = torch.tensor(index, device="cuda")
+    u = torch.tensor(uniq, device="cuda", requires_grad=True)
+    col = r.colors0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([{"params": [u], "lr": 0.05}, {"params": [col], "lr": 0.005}])
+    losses = []
+    for it in range(500):
+        opt.zero_grad()
+        img = r.render(u[idx_t], col)
+        loss = ((img - tgt) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            col.clamp_(0, 1)
+        losses.append(float(loss.detach()))
+    c0 = float((r.colors0.cpu() - torch.tensor(ct)).abs().mean())
+    c1 = float((col.detach().cpu() - torch.tensor(ct)).abs().mean())
+    assert torch.isfinite(u).all()
+    print("loss %.3f -> %.3f; colour error %.4f -> %.4f" % (losses[0], losses[-1], c0, c1))
+    # the image is recovered (most control points lie under opaque paths painted later and
+    # are not identifiable from the image, so the test does not ask for the points)
+    assert losses[-1] < 0.01 * losses[0]
+    assert c1 < 0.7 * c0
