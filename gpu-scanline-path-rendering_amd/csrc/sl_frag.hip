@@ -239,20 +239,47 @@ __global__ void k_bucket_info(uint32_t P, const int2 *__restrict__ prow, const u
     }
 }
 
+// Lanes of a wave that hit the same bucket share one atomic: the loop takes the first
+// pending lane's bucket as the leader's, serves every lane with that bucket, and repeats
+// (a wave of consecutive fragments walks a few curves, so it touches few buckets).
+__device__ __forceinline__ uint64_t lanes_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
 template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t *__restrict__ tab,
                                                      const float2 *__restrict__ tv, const int2 *__restrict__ ts, int64_t F,
                                                      int W, int H, const int2 *__restrict__ prow,
                                                      const uint32_t *__restrict__ pbase, uint32_t *bcount, uint32_t *bwsum,
                                                      uint32_t *err) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= F) return;
-    Frag f = make_fragment<SPP, false>(sc, tab, tv, ts, F, i, W, H);
-    bool ok;
-    int64_t b = bucket_of(prow[f.path], pbase[f.path], f.row, ok);
-    if (!ok) { atomicOr(err, 1u); return; }
-    if (f.row >= 0) atomicAdd(&bcount[b], 1u);
-    if (f.swn != 0) atomicAdd(&bwsum[b], (uint32_t)f.swn);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = i < F;
+    uint32_t b = 0;
+    int row = -1, swn = 0;
+    if (act) {
+        const Frag f = make_fragment<SPP, false>(sc, tab, tv, ts, F, i, W, H);
+        bool ok;
+        b = (uint32_t)bucket_of(prow[f.path], pbase[f.path], f.row, ok);
+        if (!ok) atomicOr(err, 1u);
+        act = ok;
+        row = f.row;
+        swn = f.swn;
+    }
+    uint64_t pend = __ballot(act && (row >= 0 || swn != 0));
+    while (pend) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, leader);
+        const bool mine = ((pend >> (threadIdx.x & 63)) & 1ull) && b == lb;
+        const uint64_t grp = __ballot(mine);
+        const uint32_t nc = (uint32_t)__popcll(__ballot(mine && row >= 0));
+        const int nw = (int)__popcll(__ballot(mine && swn > 0)) - (int)__popcll(__ballot(mine && swn < 0));
+        if ((threadIdx.x & 63) == leader) {
+            if (nc) atomicAdd(&bcount[lb], nc);
+            if (nw) atomicAdd(&bwsum[lb], (uint32_t)nw);
+        }
+        pend &= ~grp;
+    }
 }
 
 template <int SPP>
@@ -263,14 +290,37 @@ __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_
                                                        const uint32_t *__restrict__ boff, uint32_t *bcursor,
                                                        uint32_t *rec_meta, uint32_t *rec_mask) {
     constexpr int NW = Words<SPP>::n;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= F) return;
-    Frag f = make_fragment<SPP, true>(sc, tab, tv, ts, F, i, W, H);
-    if (f.row < 0) return;
-    bool ok;
-    int64_t b = bucket_of(prow[f.path], pbase[f.path], f.row, ok);
-    if (!ok) return;
-    const uint32_t slot = boff[b] + atomicAdd(&bcursor[b], 1u);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool act = i < F;
+    Frag f;
+    uint32_t b = 0;
+    if (act) {
+        f = make_fragment<SPP, true>(sc, tab, tv, ts, F, i, W, H);
+        bool ok;
+        b = (uint32_t)bucket_of(prow[f.path], pbase[f.path], f.row, ok);
+        act = ok && f.row >= 0;
+    }
+    // one cursor atomic per distinct bucket of the wave; each leader keeps the returned
+    // base in its own lane, and every lane reads its leader's base afterwards
+    uint64_t pend = __ballot(act);
+    uint32_t base = 0, rank = 0;
+    int my_leader = 0;
+    while (pend) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, leader);
+        const bool mine = ((pend >> lane) & 1ull) && b == lb;
+        const uint64_t grp = __ballot(mine);
+        if (mine) {
+            my_leader = leader;
+            rank = (uint32_t)__popcll(grp & lanes_lt());
+        }
+        if (lane == leader) base = atomicAdd(&bcursor[lb], (uint32_t)__popcll(grp));
+        pend &= ~grp;
+    }
+    base = (uint32_t)__shfl((int)base, my_leader, 64);
+    if (!act) return;
+    const uint32_t slot = boff[b] + base + rank;
     if (slot >= boff[b + 1]) return;
     const uint32_t cx = (uint32_t)((f.pos_x + 2) >> 1);
     rec_meta[slot] = cx | (((uint32_t)f.swn & 0xFFu) << 16) | ((f.wnc < 0 ? 1u : 0u) << 24);
