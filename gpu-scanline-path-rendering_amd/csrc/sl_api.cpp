@@ -330,11 +330,13 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         h->last_F = 0; h->last_NCH = 0; h->last_NB = 0; h->sized = true;
         return SL_OK;
     }
-    if ((rc = ensure(h, h->b_ts, (size_t)F * 8)) || (rc = ensure(h, h->b_chains, (size_t)NCH * 4)) ||
+    if ((rc = ensure(h, h->b_ts, (size_t)F * 8)) || (rc = ensure(h, h->b_chains, (size_t)NCH * 18)) ||
         (rc = ensure(h, h->b_chain_t, (size_t)F * 4)))
         return rc;
     f.ts = P_<int2>(h->b_ts);
     f.chains = P_<uint32_t>(h->b_chains);
+    f.plist = f.chains + 3 * (size_t)NCH;
+    f.lcount = P_<uint32_t>(h->b_small) + 8;
     f.chain_t = P_<float>(h->b_chain_t);
     mark(2);
     sl::launch_intersect(sc, f, s);

@@ -99,13 +99,52 @@ struct Frag {
     uint32_t m[4];
 };
 
+// Inputs of one fragment, loaded in phases so that a thread can keep the loads of several
+// fragments in flight at once: the two intersections, the curve's scalars, its vertices.
+struct FragIn {
+    int2 e0, e1;
+    uint32_t path, vp;
+    int type;
+    float w;
+    bool rev;
+    float2 a, b, d, e;
+};
+
+__device__ __forceinline__ void frag_load_ts(const int2 *__restrict__ ts, int64_t F, int64_t i, FragIn &in) {
+    in.e0 = ts[i];
+    in.e1 = (i + 1 != F) ? ts[i + 1] : make_int2(-1, 0x3f800000);
+}
+
+__device__ __forceinline__ void frag_load_curve(const SceneDev &sc, FragIn &in) {
+    const uint32_t c = (uint32_t)in.e0.x;
+    in.path = sc.cpath[c];
+    in.type = sc.ctype[c];
+    const float cw = sc.cw[c];
+    in.w = (in.type == CT_RATIONAL) ? cw : 0.f;
+    in.rev = sc.crev[c] != 0;
+    in.vp = sc.cvpos[c];
+}
+
+__device__ __forceinline__ void frag_load_verts(const float2 *__restrict__ tv, FragIn &in) {
+    const int nv = in.type & 7;
+    in.a = tv[in.vp];
+    in.b = tv[in.vp + 1];
+    in.d = nv > 2 ? tv[in.vp + 2] : make_float2(0.f, 0.f);
+    in.e = nv > 3 ? tv[in.vp + 3] : make_float2(0.f, 0.f);
+}
+
+// What the mask phase needs after the geometry phase: chord ends in cell space, the
+// vertical band and the left-edge mask.
+struct FragMaskIn {
+    float p0x, p0y, p1x, p1y;
+    uint32_t lm[4];
+    bool live;
+};
+
 template <int SPP, bool MASK>
-__device__ __forceinline__ Frag make_fragment(const SceneDev &sc, const uint32_t *__restrict__ tab,
-                                              const float2 *__restrict__ tv, const int2 *__restrict__ ts, int64_t F,
-                                              int64_t i, int W, int H) {
+__device__ __forceinline__ Frag frag_geometry(const SceneDev &sc, const FragIn &in, int W, int H, FragMaskIn &mi) {
     Frag o;
-    const int2 e0 = ts[i];
-    const int2 e1 = (i + 1 != F) ? ts[i + 1] : make_int2(-1, 0x3f800000);
+    const int2 e0 = in.e0, e1 = in.e1;
     int f0 = e0.y & 1, f1 = e1.y & 1;
     float t0 = bf((uint32_t)e0.y & 0xFFFFFFFCu);
     float t1 = bf((uint32_t)e1.y & 0xFFFFFFFCu);
@@ -113,8 +152,7 @@ __device__ __forceinline__ Frag make_fragment(const SceneDev &sc, const uint32_t
     if (t1 < 0) t1 = 0.f;
     if (t0 == 0.f || t0 == 1.f) f0 = 1;
     if (t1 == 0.f || t1 == 1.f) f1 = 1;
-    const uint32_t c = (uint32_t)e0.x;
-    o.path = sc.cpath[c];
+    o.path = in.path;
     path_canvas(sc, o.path, W, H);
     o.yx = (int32_t)0xFFFEFFFE;
     o.row = -1;
@@ -122,16 +160,13 @@ __device__ __forceinline__ Frag make_fragment(const SceneDev &sc, const uint32_t
     o.wnc = 0;
     o.swn = 0;
     o.m[0] = o.m[1] = o.m[2] = o.m[3] = 0;
+    mi.live = false;
     if (e0.x != e1.x) t1 = 1.f;
     if (!(t0 < t1)) return o;
 
-    const int type = sc.ctype[c];
-    const float w = (type == CT_RATIONAL) ? sc.cw[c] : 0.f;
-    const uint32_t vp = sc.cvpos[c];
-    const int nv = type & 7;
-    const float2 a = tv[vp], b = tv[vp + 1];
-    const float2 d = nv > 2 ? tv[vp + 2] : make_float2(0.f, 0.f);
-    const float2 e = nv > 3 ? tv[vp + 3] : make_float2(0.f, 0.f);
+    const int type = in.type;
+    const float w = in.w;
+    const float2 a = in.a, b = in.b, d = in.d, e = in.e;
     // FRAG_MASK_SUBDIV = 1: t = (1 - fi) * t0 + fi * t1 for fi = 0, 1 (.cu:211-214)
     const float ta = (1.f - 0.f) * t0 + 0.f * t1;
     const float tb = (1.f - 1.f) * t0 + 1.f * t1;
@@ -139,7 +174,7 @@ __device__ __forceinline__ Frag make_fragment(const SceneDev &sc, const uint32_t
     float p0y = curve_eval(type, w, a.y, b.y, d.y, e.y, ta);
     float p1x = curve_eval(type, w, a.x, b.x, d.x, e.x, tb);
     float p1y = curve_eval(type, w, a.y, b.y, d.y, e.y, tb);
-    const bool rev = sc.crev[c] != 0;
+    const bool rev = in.rev;
     const float vfx = rev ? p1x : p0x, vfy = rev ? p1y : p0y;
     const float vlx = rev ? p0x : p1x, vly = rev ? p0y : p1y;
     const int sf = rev ? f1 : f0, sl = rev ? f0 : f1;
@@ -187,14 +222,37 @@ __device__ __forceinline__ Frag make_fragment(const SceneDev &sc, const uint32_t
     o.wnc = wnc;
     if (MASK) {
         const float bx = (float)pos_x, by = (float)pos_y;
-        p0x = (p0x - bx) * .5f; p0y = (p0y - by) * .5f;
-        p1x = (p1x - bx) * .5f; p1y = (p1y - by) * .5f;
-        uint32_t vm[4], tm[4];
-        vmask2<SPP>(p0y, p1y, vm);
-        fetch_pixel<SPP>(tab, p0x, p0y, p1x, p1y, tm);
+        mi.p0x = (p0x - bx) * .5f; mi.p0y = (p0y - by) * .5f;
+        mi.p1x = (p1x - bx) * .5f; mi.p1y = (p1y - by) * .5f;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) o.m[k] = (tm[k] & vm[k]) ^ lm[k];
+        for (int k = 0; k < 4; ++k) mi.lm[k] = lm[k];
+        mi.live = true;
     }
+    return o;
+}
+
+// mask = pixel-table mask & vertical band ^ left-edge mask (.cu:330-371)
+template <int SPP>
+__device__ __forceinline__ void frag_mask(const uint32_t *__restrict__ tab, const FragMaskIn &mi, Frag &o) {
+    if (!mi.live) return;
+    uint32_t vm[4], tm[4];
+    vmask2<SPP>(mi.p0y, mi.p1y, vm);
+    fetch_pixel<SPP>(tab, mi.p0x, mi.p0y, mi.p1x, mi.p1y, tm);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o.m[k] = (tm[k] & vm[k]) ^ mi.lm[k];
+}
+
+template <int SPP, bool MASK>
+__device__ __forceinline__ Frag make_fragment(const SceneDev &sc, const uint32_t *__restrict__ tab,
+                                              const float2 *__restrict__ tv, const int2 *__restrict__ ts, int64_t F,
+                                              int64_t i, int W, int H) {
+    FragIn in;
+    frag_load_ts(ts, F, i, in);
+    frag_load_curve(sc, in);
+    frag_load_verts(tv, in);
+    FragMaskIn mi;
+    Frag o = frag_geometry<SPP, MASK>(sc, in, W, H, mi);
+    if (MASK) frag_mask<SPP>(tab, mi, o);
     return o;
 }
 
@@ -247,38 +305,79 @@ __device__ __forceinline__ uint64_t lanes_lt() {
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
+// Fragments per thread of the count and scatter passes.  A thread's fragments are 64
+// apart (lane-major), so each of its K fragments is coalesced across the wave, and their
+// loads are issued phase by phase to overlap the dependent memory round trips.
+constexpr int kFragPerThread = 2;
+
+template <int K>
+__device__ __forceinline__ void load_fragments(const SceneDev &sc, const float2 *__restrict__ tv,
+                                               const int2 *__restrict__ ts, int64_t F, int64_t i0,
+                                               const int2 *__restrict__ prow, const uint32_t *__restrict__ pbase,
+                                               FragIn *in, bool *live, int2 *pr, uint32_t *pb) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        live[j] = i0 + 64 * j < F;
+        if (live[j]) frag_load_ts(ts, F, i0 + 64 * j, in[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (live[j]) frag_load_curve(sc, in[j]);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (live[j]) {
+            frag_load_verts(tv, in[j]);
+            pr[j] = prow[in[j].path];
+            pb[j] = pbase[in[j].path];
+        }
+    }
+}
+
 template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t *__restrict__ tab,
                                                      const float2 *__restrict__ tv, const int2 *__restrict__ ts, int64_t F,
                                                      int W, int H, const int2 *__restrict__ prow,
                                                      const uint32_t *__restrict__ pbase, uint32_t *bcount, uint32_t *bwsum,
                                                      uint32_t *err) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool act = i < F;
-    uint32_t b = 0;
-    int row = -1, swn = 0;
-    if (act) {
-        const Frag f = make_fragment<SPP, false>(sc, tab, tv, ts, F, i, W, H);
-        bool ok;
-        b = (uint32_t)bucket_of(prow[f.path], pbase[f.path], f.row, ok);
-        if (!ok) atomicOr(err, 1u);
-        act = ok;
-        row = f.row;
-        swn = f.swn;
-    }
-    uint64_t pend = __ballot(act && (row >= 0 || swn != 0));
-    while (pend) {
-        const int leader = __ffsll((long long)pend) - 1;
-        const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, leader);
-        const bool mine = ((pend >> (threadIdx.x & 63)) & 1ull) && b == lb;
-        const uint64_t grp = __ballot(mine);
-        const uint32_t nc = (uint32_t)__popcll(__ballot(mine && row >= 0));
-        const int nw = (int)__popcll(__ballot(mine && swn > 0)) - (int)__popcll(__ballot(mine && swn < 0));
-        if ((threadIdx.x & 63) == leader) {
-            if (nc) atomicAdd(&bcount[lb], nc);
-            if (nw) atomicAdd(&bwsum[lb], (uint32_t)nw);
+    constexpr int K = kFragPerThread;
+    const int lane = threadIdx.x & 63;
+    const int64_t i0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64 * K + lane;
+    FragIn in[K];
+    bool live[K];
+    int2 pr[K];
+    uint32_t pb[K];
+    load_fragments<K>(sc, tv, ts, F, i0, prow, pbase, in, live, pr, pb);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        bool act = live[j];
+        uint32_t b = 0;
+        int row = -1, swn = 0;
+        if (act) {
+            FragMaskIn mi;
+            const Frag f = frag_geometry<SPP, false>(sc, in[j], W, H, mi);
+            bool ok;
+            b = (uint32_t)bucket_of(pr[j], pb[j], f.row, ok);
+            if (!ok) atomicOr(err, 1u);
+            act = ok;
+            row = f.row;
+            swn = f.swn;
         }
-        pend &= ~grp;
+        // lanes of the wave with the same bucket share one atomic (the wave walks a few
+        // curves, so it touches few buckets)
+        uint64_t pend = __ballot(act && (row >= 0 || swn != 0));
+        while (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, leader);
+            const bool mine = ((pend >> lane) & 1ull) && b == lb;
+            const uint64_t grp = __ballot(mine);
+            const uint32_t nc = (uint32_t)__popcll(__ballot(mine && row >= 0));
+            const int nw = (int)__popcll(__ballot(mine && swn > 0)) - (int)__popcll(__ballot(mine && swn < 0));
+            if (lane == leader) {
+                if (nc) atomicAdd(&bcount[lb], nc);
+                if (nw) atomicAdd(&bwsum[lb], (uint32_t)nw);
+            }
+            pend &= ~grp;
+        }
     }
 }
 
@@ -290,46 +389,65 @@ __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_
                                                        const uint32_t *__restrict__ boff, uint32_t *bcursor,
                                                        uint32_t *rec_meta, uint32_t *rec_mask) {
     constexpr int NW = Words<SPP>::n;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int K = kFragPerThread;
     const int lane = threadIdx.x & 63;
-    bool act = i < F;
-    Frag f;
-    uint32_t b = 0;
-    if (act) {
-        f = make_fragment<SPP, true>(sc, tab, tv, ts, F, i, W, H);
-        bool ok;
-        b = (uint32_t)bucket_of(prow[f.path], pbase[f.path], f.row, ok);
-        act = ok && f.row >= 0;
-    }
-    // one cursor atomic per distinct bucket of the wave; each leader keeps the returned
-    // base in its own lane, and every lane reads its leader's base afterwards
-    uint64_t pend = __ballot(act);
-    uint32_t base = 0, rank = 0;
-    int my_leader = 0;
-    while (pend) {
-        const int leader = __ffsll((long long)pend) - 1;
-        const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, leader);
-        const bool mine = ((pend >> lane) & 1ull) && b == lb;
-        const uint64_t grp = __ballot(mine);
-        if (mine) {
-            my_leader = leader;
-            rank = (uint32_t)__popcll(grp & lanes_lt());
+    const int64_t i0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64 * K + lane;
+    FragIn in[K];
+    bool live[K];
+    int2 pr[K];
+    uint32_t pb[K];
+    load_fragments<K>(sc, tv, ts, F, i0, prow, pbase, in, live, pr, pb);
+    Frag f[K];
+    FragMaskIn mi[K];
+    bool act[K];
+    uint32_t b[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        act[j] = false;
+        b[j] = 0;
+        if (live[j]) {
+            f[j] = frag_geometry<SPP, true>(sc, in[j], W, H, mi[j]);
+            bool ok;
+            b[j] = (uint32_t)bucket_of(pr[j], pb[j], f[j].row, ok);
+            act[j] = ok && f[j].row >= 0;
         }
-        if (lane == leader) base = atomicAdd(&bcursor[lb], (uint32_t)__popcll(grp));
-        pend &= ~grp;
     }
-    base = (uint32_t)__shfl((int)base, my_leader, 64);
-    if (!act) return;
-    const uint32_t slot = boff[b] + base + rank;
-    if (slot >= boff[b + 1]) return;
-    const uint32_t cx = (uint32_t)((f.pos_x + 2) >> 1);
-    rec_meta[slot] = cx | (((uint32_t)f.swn & 0xFFu) << 16) | ((f.wnc < 0 ? 1u : 0u) << 24);
-    if (NW == 4) {
-        *(uint4 *)(rec_mask + (size_t)slot * 4) = make_uint4(f.m[0], f.m[1], f.m[2], f.m[3]);
-    } else if (NW == 2) {
-        *(uint2 *)(rec_mask + (size_t)slot * 2) = make_uint2(f.m[0], f.m[1]);
-    } else {
-        rec_mask[slot] = f.m[0];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (act[j]) frag_mask<SPP>(tab, mi[j], f[j]);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        // one cursor atomic per distinct bucket of the wave; each leader keeps the returned
+        // base in its own lane, and every lane reads its leader's base afterwards
+        uint64_t pend = __ballot(act[j]);
+        uint32_t base = 0, rank = 0;
+        int my_leader = 0;
+        while (pend) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b[j], leader);
+            const bool mine = ((pend >> lane) & 1ull) && b[j] == lb;
+            const uint64_t grp = __ballot(mine);
+            if (mine) {
+                my_leader = leader;
+                rank = (uint32_t)__popcll(grp & lanes_lt());
+            }
+            if (lane == leader) base = atomicAdd(&bcursor[lb], (uint32_t)__popcll(grp));
+            pend &= ~grp;
+        }
+        base = (uint32_t)__shfl((int)base, my_leader, 64);
+        if (!act[j]) continue;
+        const uint32_t slot = boff[b[j]] + base + rank;
+        if (slot >= boff[b[j] + 1]) continue;
+        const Frag &g = f[j];
+        const uint32_t cx = (uint32_t)((g.pos_x + 2) >> 1);
+        rec_meta[slot] = cx | (((uint32_t)g.swn & 0xFFu) << 16) | ((g.wnc < 0 ? 1u : 0u) << 24);
+        if (NW == 4) {
+            *(uint4 *)(rec_mask + (size_t)slot * 4) = make_uint4(g.m[0], g.m[1], g.m[2], g.m[3]);
+        } else if (NW == 2) {
+            *(uint2 *)(rec_mask + (size_t)slot * 2) = make_uint2(g.m[0], g.m[1]);
+        } else {
+            rec_mask[slot] = g.m[0];
+        }
     }
 }
 
@@ -365,7 +483,7 @@ void launch_bucket_info(const SceneDev &sc, Frame &f, hipStream_t s) {
 void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {
     if (!f.F) return;
     const unsigned fb = f.frag_block > 0 && f.frag_block <= 256 ? (unsigned)f.frag_block : 256u;
-    dim3 g(blocks_for(f.F, fb)), b(fb);
+    dim3 g(blocks_for(f.F, fb * kFragPerThread)), b(fb);
 #define SL_LAUNCH(N)                                                                                              \
     hipLaunchKernelGGL(k_frag_count<N>, g, b, 0, s, sc, t.pixel, (const float2 *)f.tvtx, (const int2 *)f.ts, f.F, \
                        f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, f.bcount, f.bwsum, f.err_flag)
@@ -378,7 +496,7 @@ void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_
 void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {
     if (!f.F) return;
     const unsigned fb = f.frag_block > 0 && f.frag_block <= 256 ? (unsigned)f.frag_block : 256u;
-    dim3 g(blocks_for(f.F, fb)), b(fb);
+    dim3 g(blocks_for(f.F, fb * kFragPerThread)), b(fb);
 #define SL_LAUNCH(N)                                                                                                \
     hipLaunchKernelGGL(k_frag_scatter<N>, g, b, 0, s, sc, t.pixel, (const float2 *)f.tvtx, (const int2 *)f.ts, f.F, \
                        f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, (const uint32_t *)f.boff,         \

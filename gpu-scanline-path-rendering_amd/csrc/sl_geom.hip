@@ -248,9 +248,39 @@ __device__ __forceinline__ float pick5(int i, float a, float b, float c, float d
 // k_make_intersection_0 (ras_cut.h:1254-1482): monotone cuts, then per monotone piece the
 // numbers of even x / y cut lines it crosses (d_udpate_cut_range) and the curve's fragment
 // count.  Also the number of solve chains (two per piece) of the intersection pass.
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = tmin(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = tmax(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+// Range of cell rows a visible curve's fragments can fall in: a fragment's row comes from
+// its chord midpoint, which lies inside the control-point hull (rationals with w > 0), so
+// the hull's rows with one row of margin for rounding, clamped to the canvas rows.
+__device__ __forceinline__ void curve_rows(const Curve &cu, int H, int &rlo, int &rhi) {
+    const int nv = cu.type & 7;
+    float lo = fminf(cu.y0, cu.y1), hi = fmaxf(cu.y0, cu.y1);
+    if (nv > 2) { lo = fminf(lo, cu.y2); hi = fmaxf(hi, cu.y2); }
+    if (nv > 3) { lo = fminf(lo, cu.y3); hi = fmaxf(hi, cu.y3); }
+    const int rmax = (H - 1) >> 1;
+    const bool hull = cu.type != CT_RATIONAL || cu.w > 0.f;
+    if (hull && fabsf(lo) < 1e30f && fabsf(hi) < 1e30f) {
+        rlo = tmax((floor_i32(lo) >> 1) - 1, 0);
+        rhi = tmin((floor_i32(hi) >> 1) + 1, rmax);
+    } else {
+        rlo = 0;
+        rhi = rmax;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__restrict__ tv, const uint32_t *__restrict__ pflags,
                                                    int W, int H, float4 *ccuts, uint8_t *cncut, uint32_t *cfrag,
-                                                   uint32_t *pcnt, uint32_t *nchain) {
+                                                   uint32_t *pcnt, uint32_t *nchain, int2 *prow) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= sc.C) return;
     Curve cu = load_curve(sc, tv, c);
@@ -306,15 +336,89 @@ __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__r
     for (int k = 0; k < 5; ++k) pcnt[5 * (size_t)c + k] = pc[k];
     cfrag[c] = cnt;
     nchain[c] = 2u * (uint32_t)npieces;
+    // the path's row range, one atomic pair per distinct path of the wave
+    int rlo = 0, rhi = -1;
+    if (vis) curve_rows(cu, H, rlo, rhi);
+    const uint32_t path = sc.cpath[c];
+    const int lane = threadIdx.x & 63;
+    uint64_t pend = __ballot(vis && rlo <= rhi);
+    while (pend) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const uint32_t lp = (uint32_t)__builtin_amdgcn_readlane((int)path, leader);
+        const bool mine = ((pend >> lane) & 1ull) && path == lp;
+        const uint64_t grp = __ballot(mine);
+        const int mn = wave_min_i(mine ? rlo : INT_MAX), mx = wave_max_i(mine ? rhi : INT_MIN);
+        if (lane == leader) {
+            atomicMin(&prow[lp].x, mn);
+            atomicMax(&prow[lp].y, mx);
+        }
+        pend &= ~grp;
+    }
 }
 
-// one descriptor per (curve, piece, side) solve chain: curve << 4 | piece << 1 | side
-__global__ void k_chain_list(uint32_t C, const uint32_t *__restrict__ nchain, const uint32_t *__restrict__ choff,
-                             uint32_t *chains) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    const uint32_t n = nchain[c], o = choff[c];
-    for (uint32_t k = 0; k < n; ++k) chains[o + k] = (c << 4) | k;
+// Solve chains (curve << 4 | piece << 1 | side, non-empty only) and merge pieces
+// (curve << 3 | piece) listed in three length classes, longest first: the sequential
+// walks of long chains start early, and each wave's lanes carry similar lengths.
+__device__ __forceinline__ int len_class(uint32_t L) { return L >= 128 ? 0 : (L >= 16 ? 1 : 2); }
+
+__global__ void __launch_bounds__(256) k_chain_list(uint32_t C, const uint32_t *__restrict__ nchain,
+                                                    const uint32_t *__restrict__ pcnt, uint32_t cstride,
+                                                    uint32_t pstride, uint32_t *clist, uint32_t *plist,
+                                                    uint32_t *counts /* [3] chains, [3] pieces */) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool live = c < C;
+    const int np = live ? (int)(nchain[c] >> 1) : 0;
+    uint32_t pc[5] = {0u, 0u, 0u, 0u, 0u};
+    for (int k = 0; k < np; ++k) pc[k] = pcnt[5 * (size_t)c + k];
+    // per-thread counts per class, one reservation per class and wave
+    uint32_t n[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    for (int k = 0; k < np; ++k) {
+        const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
+        if (nx) n[len_class(nx)]++;
+        if (ny) n[len_class(ny)]++;
+        n[3 + len_class(nx + ny + 1)]++;
+    }
+    uint32_t off[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        uint32_t inc = n[q];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        const uint32_t tot = (uint32_t)__shfl((int)inc, 63, 64);
+        uint32_t base = 0;
+        if (lane == 63 && tot) base = atomicAdd(&counts[q], tot);
+        base = (uint32_t)__shfl((int)base, 63, 64);
+        off[q] = base + inc - n[q];
+    }
+    for (int k = 0; k < np; ++k) {
+        const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
+        if (nx) {
+            const int q = len_class(nx);
+            clist[(size_t)q * cstride + off[q]++] = (c << 4) | ((uint32_t)k << 1);
+        }
+        if (ny) {
+            const int q = len_class(ny);
+            clist[(size_t)q * cstride + off[q]++] = (c << 4) | ((uint32_t)k << 1) | 1u;
+        }
+        const int q = len_class(nx + ny + 1);
+        plist[(size_t)q * pstride + off[3 + q]++] = (c << 3) | (uint32_t)k;
+    }
+}
+
+// entry g of the class lists, in class order; false past their end
+__device__ __forceinline__ bool class_entry(uint32_t g, const uint32_t *__restrict__ list,
+                                            const uint32_t *__restrict__ cnt, uint32_t stride, uint32_t &d) {
+    const uint32_t n0 = cnt[0], n1 = cnt[1], n2 = cnt[2];
+    if (g < n0) { d = list[g]; return true; }
+    g -= n0;
+    if (g < n1) { d = list[stride + g]; return true; }
+    g -= n1;
+    if (g < n2) { d = list[2 * (size_t)stride + g]; return true; }
+    return false;
 }
 
 // t | flags of a monotone piece's end: low bits 2 when the end lies on an integral x,
@@ -356,11 +460,12 @@ __device__ __forceinline__ float solve_cut(int type, float w, float a0, float a1
 // other side's: solve it sequentially, storing t | side for the merge pass.
 __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
                                                  const uint8_t *__restrict__ cncut, const uint32_t *__restrict__ coff,
-                                                 const uint32_t *__restrict__ pcnt, const uint32_t *__restrict__ chains,
-                                                 const uint32_t *nch_ptr, int W, int H, float *chain_t) {
+                                                 const uint32_t *__restrict__ pcnt, const uint32_t *__restrict__ clist,
+                                                 const uint32_t *__restrict__ counts, uint32_t cstride, int W, int H,
+                                                 float *chain_t) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *nch_ptr) return;
-    const uint32_t d = chains[i];
+    uint32_t d;
+    if (!class_entry(i, clist, counts, cstride, d)) return;
     const uint32_t c = d >> 4;
     const int k = (int)((d >> 1) & 7u), side = (int)(d & 1u);
     const uint32_t *pc = pcnt + 5 * (size_t)c;
@@ -399,87 +504,62 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
     }
 }
 
-// Intersection pass 2: one thread per curve replays the reference's slot walk
-// (ras_cut.h:910-1000) over the solved chains: pick the x slot when x <= y, emit its t,
-// refill it from the chain or with the exhausted marker 2 | side; equal consecutive t's
-// OR their flags into the previous entry.  It also records the range of cell rows of the
-// curve's fragments (chord midpoints, .cu:247-262) for the (path, row) bucketing.
+// Intersection pass 2: one thread per monotone piece replays the reference's slot walk
+// (ras_cut.h:910-1000) over the piece's solved chains: pick the x slot when x <= y, emit
+// its t, refill it from the chain or with the exhausted marker 2 | side; equal consecutive
+// t's OR their flags into the previous entry.  A piece starts at the previous piece's end
+// flags (t = 0 for the first), exactly as the per-curve walk carries them.
 __global__ void __launch_bounds__(256) k_merge(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
                                                 const uint8_t *__restrict__ cncut, const uint32_t *__restrict__ coff,
-                                                const uint32_t *__restrict__ pcnt, const float *__restrict__ chain_t, int H,
-                                                int2 *ts, int2 *prow) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= sc.C) return;
-    const uint32_t base = coff[c];
-    if (coff[c + 1] == base) return;
-    int W = 0;
-    path_canvas(sc, sc.cpath[c], W, H);
-    Curve cu = load_curve(sc, tv, c);
+                                                const uint32_t *__restrict__ pcnt, const float *__restrict__ chain_t,
+                                                const uint32_t *__restrict__ plist, const uint32_t *__restrict__ counts,
+                                                uint32_t pstride, int2 *ts) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t d;
+    if (!class_entry(g, plist, counts, pstride, d)) return;
+    const uint32_t c = d >> 3;
+    const int k = (int)(d & 7u);
+    const Curve cu = load_curve(sc, tv, c);
     const float4 q = ccuts[c];
     const int n = cncut[c];
     const uint32_t *pc = pcnt + 5 * (size_t)c;
-    int2 *out = ts + base;
-    const float *tt = chain_t + base;
-    int rmin = INT_MAX, rmax = INT_MIN;
-    bool have_prev = false;
-    float prev_t = 0.f, prev_y = 0.f;
-    auto track = [&](float t) {
-        const float y = ey(cu, t);
-        if (have_prev && prev_t < t) {
-            const int32_t ry = floor_even((prev_y + y) * 0.5f);
-            if ((uint32_t)ry < (uint32_t)H) {
-                rmin = tmin(rmin, ry >> 1);
-                rmax = tmax(rmax, ry >> 1);
-            }
-        }
-        have_prev = true;
-        prev_t = t;
-        prev_y = y;
-    };
-    const float two = 2.f, two1 = bf(fb(2.f) | 1u);
     uint32_t p = 0;
+    for (int kk = 0; kk < k; ++kk) p += 1u + (pc[kk] & 0xFFFFu) + (pc[kk] >> 16);
+    const uint32_t base = coff[c] + p;
+    int2 *out = ts + base;
+    const float *X = chain_t + base;
+    const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
+    const float *Y = X + nx;
     float t0m = 0.f;
-    for (int k = 0; k <= n; ++k) {
-        const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
-        const float t1r = (k == n) ? 1.f : pick5(k, q.x, q.y, q.z, q.w, 1.f);
-        const float t1m = piece_end_flags(t1r, ex(cu, t1r));
-        const float *X = tt + p;
-        const float *Y = X + nx;
-        float xs, ys;
-        uint32_t xi = 0, yi = 0, rx, ry;
-        if (nx == 0) { xs = two; ys = t0m; rx = 0; ry = ny; }
-        else if (ny == 0) { xs = t0m; ys = two1; rx = nx; ry = 0; }
-        else { xs = X[0]; xi = 1; rx = nx - 1; ys = t0m; ry = ny; }
-        uint32_t last = fb(-1.f);
-        const uint32_t n_loop = nx + ny + 1;
-        for (uint32_t it = 0; it < n_loop; ++it) {
-            float v;
-            if (xs <= ys) {
-                v = xs;
-                if (rx > 0) { xs = X[xi++]; rx--; } else { xs = two; }
-            } else {
-                v = ys;
-                if (ry > 0) { ys = Y[yi++]; ry--; } else { ys = two1; }
-            }
-            uint32_t o = fb(v);
-            if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
-                o |= last;
-                out[p - 1].y = (int)o;
-            }
-            out[p] = make_int2((int)c, (int)o);
-            last = o;
-            p++;
-            float tm = bf(o & 0xFFFFFFFCu);
-            if (tm < 0.f) tm = 0.f;
-            track(tm);
-        }
-        t0m = t1m;
+    if (k > 0) {
+        const float tr = pick5(k - 1, q.x, q.y, q.z, q.w, 1.f);
+        t0m = piece_end_flags(tr, ex(cu, tr));
     }
-    track(1.f);  // the curve's last fragment ends at t = 1 (.cu:148)
-    if (rmin <= rmax) {
-        const uint32_t pth = sc.cpath[c];
-        atomicMin(&prow[pth].x, rmin);
-        atomicMax(&prow[pth].y, rmax);
+    (void)n;
+    const float two = 2.f, two1 = bf(fb(2.f) | 1u);
+    float xs, ys;
+    uint32_t xi = 0, yi = 0, rx, ry;
+    if (nx == 0) { xs = two; ys = t0m; rx = 0; ry = ny; }
+    else if (ny == 0) { xs = t0m; ys = two1; rx = nx; ry = 0; }
+    else { xs = X[0]; xi = 1; rx = nx - 1; ys = t0m; ry = ny; }
+    uint32_t last = fb(-1.f);
+    const uint32_t n_loop = nx + ny + 1;
+    for (uint32_t it = 0; it < n_loop; ++it) {
+        float v;
+        if (xs <= ys) {
+            v = xs;
+            if (rx > 0) { xs = X[xi++]; rx--; } else { xs = two; }
+        } else {
+            v = ys;
+            if (ry > 0) { ys = Y[yi++]; ry--; } else { ys = two1; }
+        }
+        uint32_t o = fb(v);
+        if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
+            o |= last;
+            out[it - 1].y = (int)o;
+        }
+        out[it] = make_int2((int)c, (int)o);
+        last = o;
     }
 }
 
@@ -523,20 +603,25 @@ void launch_transform(const SceneDev &sc, Frame &f, hipStream_t s) {
 void launch_monotone(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.C) return;
     hipLaunchKernelGGL(k_monotone, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                       (const uint32_t *)f.pflags, f.W, f.H, f.ccuts, f.cncut, f.cfrag, f.pcnt, f.nchain);
+                       (const uint32_t *)f.pflags, f.W, f.H, f.ccuts, f.cncut, f.cfrag, f.pcnt, f.nchain, f.prow);
 }
 
 void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.C) return;
+    const uint32_t cstride = (uint32_t)f.NCH, pstride = (uint32_t)(f.NCH / 2);
+    launch_fill_u32(f.lcount, 0u, 6, s);
     hipLaunchKernelGGL(k_chain_list, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc.C, (const uint32_t *)f.nchain,
-                       (const uint32_t *)f.choff, f.chains);
-    const uint32_t *nch = f.choff + sc.C;
-    hipLaunchKernelGGL(k_chains, dim3(blocks_for(f.NCH ? f.NCH : 1, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                       (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
-                       (const uint32_t *)f.pcnt, (const uint32_t *)f.chains, nch, f.W, f.H, f.chain_t);
-    hipLaunchKernelGGL(k_merge, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                       (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
-                       (const uint32_t *)f.pcnt, (const float *)f.chain_t, f.H, f.ts, f.prow);
+                       (const uint32_t *)f.pcnt, cstride, pstride, f.chains, f.plist, f.lcount);
+    if (f.NCH)
+        hipLaunchKernelGGL(k_chains, dim3(blocks_for(f.NCH, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
+                           (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
+                           (const uint32_t *)f.pcnt, (const uint32_t *)f.chains, (const uint32_t *)f.lcount, cstride,
+                           f.W, f.H, f.chain_t);
+    if (pstride)
+        hipLaunchKernelGGL(k_merge, dim3(blocks_for(pstride, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
+                           (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
+                           (const uint32_t *)f.pcnt, (const float *)f.chain_t, (const uint32_t *)f.plist,
+                           (const uint32_t *)(f.lcount + 3), pstride, f.ts);
 }
 
 }  // namespace sl

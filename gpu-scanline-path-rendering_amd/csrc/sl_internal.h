@@ -61,7 +61,9 @@ struct Frame {
     uint32_t *pcnt;          // [5C] per monotone piece: x cuts | y cuts << 16
     uint32_t *nchain;        // [C+1] solve chains per curve (2 per piece)
     uint32_t *choff;         // [C+1] exclusive scan of nchain
-    uint32_t *chains;        // [NCH] chain descriptors
+    uint32_t *chains;        // [3][NCH] solve chain descriptors in length classes
+    uint32_t *plist;         // [3][NCH/2] merge piece descriptors in length classes
+    uint32_t *lcount;        // [6] class list lengths: chains, then pieces
     int64_t NCH;
     float *chain_t;          // [F] solved t | side per chain
     // fragments
