@@ -279,25 +279,157 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
 
 constexpr int kResolveBlocks = 16384;
 
+// One small bucket's inputs, loaded a bucket ahead of its resolve (the grid-stride loop
+// keeps the next bucket's records and scalars in flight while the current one resolves;
+// offsets run two buckets ahead because the records' address depends on them).
+template <int NW>
+struct SmallIn {
+    uint32_t meta, mk[NW];
+    uint32_t carry;
+    int rule, W;
+};
+
+template <int NW>
+__device__ __forceinline__ void small_load(const ResolveArgs &A, int64_t b, uint32_t base, uint32_t end, int2 info,
+                                           SmallIn<NW> &o) {
+    const int lane = threadIdx.x & 63;
+    const int n = (int)(end - base);
+    o.meta = 0u;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) o.mk[k] = 0u;
+    if (n > 0 && n <= 64 && lane < n) {
+        o.meta = A.rec_meta[base + lane];
+        load_mask<NW>(A.rec_mask, (size_t)base + lane, o.mk);
+    }
+    o.rule = A.frule[info.x];
+    o.carry = A.bcarry[b];
+    o.W = A.W;
+    if (A.pcanvas) {
+        const int4 cv = A.pcanvas[info.x];
+        o.W = cv.x;
+        o.carry -= A.scarry[cv.w];
+    }
+}
+
+// resolve_bucket<SPP, 1> with the records already in registers
+template <int SPP>
+__device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, uint32_t base, int n,
+                                              const SmallIn<Words<SPP>::n> &in, uint32_t *order, uint32_t *meta,
+                                              uint32_t *msk) {
+    constexpr int NW = Words<SPP>::n;
+    const int lane = threadIdx.x & 63;
+    uint32_t mykey = 0xFFFFFFFFu;
+    if (lane < n) {
+        meta[lane] = in.meta;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) msk[lane * NW + k] = in.mk[k];
+        mykey = ((in.meta & 0xFFFFu) << 10) | (uint32_t)lane;
+    }
+    int rk = 0;
+    for (int j = 0; j < n; ++j) {
+        const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)mykey, j);
+        rk += (kj < mykey) ? 1 : 0;
+    }
+    if (lane < n) order[rk] = (uint32_t)lane;
+    __syncthreads();
+
+    const int rule = in.rule;
+    uint32_t wcarry = in.carry;
+    const int W = in.W;
+    const size_t ib = 2 * (size_t)base;
+    const int p = lane;
+    const bool v = p < n;
+    const uint32_t i = v ? order[p] : 0u;
+    const uint32_t m = v ? meta[i] : 0u;
+    const int cx = (int)(m & 0xFFFFu);
+    const int swn = v ? (int)(int8_t)(m >> 16) : 0;
+    const int prev_cx = (v && p > 0) ? (int)(meta[order[p - 1]] & 0xFFFFu) : -1;
+    const bool head = v && (p == 0 || cx != prev_cx);
+    const int incl = scan_add(swn);
+    const int8_t w_at = (int8_t)(int)(wcarry + (uint32_t)(incl - swn));
+    const int x = 2 * cx - 2;
+    const bool merged = head && x >= 0 && x < W;
+    uint32_t res[NW];
+    if (merged) {
+        Stencil<NW> st;
+        st.init(w_at);
+        for (int q = p; q < n; ++q) {
+            const uint32_t iq = order[q];
+            const uint32_t mq = meta[iq];
+            if ((int)(mq & 0xFFFFu) != cx) break;
+            st.add(msk + iq * NW, ((mq >> 24) & 1u) != 0u);
+        }
+        st.finish(rule, res);
+    }
+    const int prev_x = 2 * prev_cx - 2;
+    const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
+    const int cnt = (merged ? 1 : 0) + (span ? 1 : 0);
+    const int ic = scan_add(cnt);
+    const int idx = ic - cnt;
+    if (span) {
+        const int x0 = prev_x + 2;
+        A.items[ib + idx] = (uint32_t)x0 | ((uint32_t)(x - x0) << 16);
+    }
+    if (merged) {
+        A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
+        store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
+        A.icov[ib + idx + (span ? 1 : 0)] = cell_counts<SPP>(res);
+    }
+    const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
+    const int hi = merged ? x + 2 : (span ? x : INT_MIN);
+    const int xmin = lane63(dpp_scan(lo, INT_MAX, OpMin()));
+    const int xmax = lane63(dpp_scan(hi, INT_MIN, OpMax()));
+    const int nmerged = lane63(scan_add(merged ? 1 : 0));
+    const int icnt = lane63(ic);
+    if (lane == 0) {
+        A.bicnt[b] = (uint32_t)icnt;
+        A.bmcnt[b] = (uint32_t)nmerged;
+        A.bext[b] = icnt ? ((uint32_t)xmin | ((uint32_t)xmax << 16)) : 0xFFFFu;
+    }
+    __syncthreads();
+}
+
 // Every bucket: empty ones are cleared, buckets of at most 64 fragments are resolved,
 // larger ones are listed for the size-class passes below.
 template <int SPP>
 __global__ void __launch_bounds__(64) k_resolve_first(ResolveArgs A) {
     constexpr int NW = Words<SPP>::n;
-    __shared__ uint32_t key[64], order[64], meta[64], msk[64 * NW];
+    __shared__ uint32_t order[64], meta[64], msk[64 * NW];
     const int lane = threadIdx.x;
-    for (int64_t b = blockIdx.x; b < A.NB; b += gridDim.x) {
-        const uint32_t base = A.boff[b];
-        const int n = (int)(A.boff[b + 1] - base);
+    const int64_t G = gridDim.x;
+    int64_t b = blockIdx.x;
+    if (b >= A.NB) return;
+    auto head = [&](int64_t bb, uint32_t &base, uint32_t &end, int2 &info) {
+        if (bb < A.NB) {
+            base = A.boff[bb];
+            end = A.boff[bb + 1];
+            info = A.binfo[bb];
+        } else {
+            base = end = 0u;
+            info = make_int2(0, 0);
+        }
+    };
+    uint32_t base0, end0, base1, end1;
+    int2 info0, info1;
+    head(b, base0, end0, info0);
+    head(b + G, base1, end1, info1);
+    SmallIn<NW> in0, in1;
+    small_load<NW>(A, b, base0, end0, info0, in0);
+    for (; b < A.NB; b += G) {
+        uint32_t base2, end2;
+        int2 info2;
+        head(b + 2 * G, base2, end2, info2);
+        if (b + G < A.NB) small_load<NW>(A, b + G, base1, end1, info1, in1);
+        const int n = (int)(end0 - base0);
         if (n == 0) {
             if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; }
-            continue;
-        }
-        if (n > 64) {
+        } else if (n > 64) {
             if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)b;
-            continue;
+        } else {
+            resolve_small<SPP>(A, b, base0, n, in0, order, meta, msk);
         }
-        resolve_bucket<SPP, 1>(A, b, base, n, key, order, meta, msk);
+        base0 = base1; end0 = end1; info0 = info1; in0 = in1;
+        base1 = base2; end1 = end2; info1 = info2;
     }
 }
 
