@@ -52,12 +52,12 @@ def b_alg_total(cnt, spp, W, H):
             + M * 2 * m + S * 2 * 8 + W * H * 4)
 
 
-def cpu_baseline(scene_full, W, H, spp, sample_paths, seed):
+def cpu_baseline(scene_full, W, H, spp, sample_paths, seed, kind):
     """OpenMP oracle on the first `sample_paths` paths of the same scene generator."""
     import pyoracle
     import scanline
 
-    sub = scanline.generate(scanline.scene.KIND_MIXED, W, H, sample_paths, seed)
+    sub = scanline.generate(kind, W, H, sample_paths, seed)
     L = pyoracle.lib(omp=True)
     threads = L.oracle_threads()
     times, frags = [], 0
@@ -247,6 +247,8 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
     ap.add_argument("--mode", choices=["plain", "batch", "diff"], default="plain")
+    ap.add_argument("--config", type=int, choices=[2, 3], default=2,
+                    help="plain mode workload: 2 = 2048^2 mixed paths (headline), 3 = text-like 4096^2")
     ap.add_argument("--batch-scenes", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -265,9 +267,20 @@ def main():
 
     import scanline
 
-    W = H = args.size
-    seed = scanline.scene.SEED_BASE ^ 2
-    scene = scanline.generate(scanline.scene.KIND_MIXED, W, H, args.paths, seed)
+    if args.config == 3:
+        # text-like workload: 4096x4096, 100,000 glyphs (1-4 contours of 10-40 quadratics,
+        # reversed holes, nonzero), 16 spp
+        kind, seed = scanline.scene.KIND_GLYPHS, scanline.scene.SEED_BASE ^ 3
+        W = H = 4096
+        args.paths, args.spp = 100000, 16
+        workload = "text-like scene, 4096x4096 canvas, 100k glyphs, 16 spp"
+        cpu_paths = args.cpu_sample_paths * 5
+    else:
+        kind, seed = scanline.scene.KIND_MIXED, scanline.scene.SEED_BASE ^ 2
+        W = H = args.size
+        workload = "plain renderer, %dx%d canvas, %d mixed paths, %d spp" % (W, H, args.paths, args.spp)
+        cpu_paths = args.cpu_sample_paths
+    scene = scanline.generate(kind, W, H, args.paths, seed)
     r = scanline.VGRasterizer(0)
     r.setSamples(args.spp)
     r.setMultisampleOutput(args.ms)
@@ -308,7 +321,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, W, H, args.spp, args.cpu_sample_paths, seed)
+        cpu = cpu_baseline(scene, W, H, args.spp, cpu_paths, seed, kind)
 
     out = {
         "metric": "Mfragments/s + Mpixels/s at 2048^2 32spp (GB/s vs HBM peak); bwd/fwd time",
@@ -323,7 +336,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, splitmix64 seed 0x5CA1AB1E^2)",
-        "config": {"workload": "plain renderer, 2048x2048 canvas, 20k mixed paths, 32 spp",
+        "config": {"workload": workload,
                    "width": W, "height": H, "paths": args.paths, "spp": args.spp,
                    "output": "multisample" if args.ms else "coverage", "srgb": True},
         "mpixels_per_s": round(W * H / (ms_step * 1e-3) / 1e6, 3),

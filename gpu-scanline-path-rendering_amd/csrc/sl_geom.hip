@@ -116,14 +116,15 @@ struct Mat4 { float m[16]; };
 __global__ void k_transform(uint32_t V, const float2 *__restrict__ vtx, const uint32_t *__restrict__ vpath,
                             const int4 *__restrict__ pcanvas, Mat4 M, int W, int H, float2 *__restrict__ out,
                             uint32_t *pflags) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= V) return;
-    if (pcanvas) {
-        const int4 cv = pcanvas[vpath[i]];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < V;
+    const uint32_t p = live ? vpath[i] : 0u;
+    if (pcanvas && live) {
+        const int4 cv = pcanvas[p];
         W = cv.x;
         H = cv.y;
     }
-    float2 iv = vtx[i];
+    const float2 iv = live ? vtx[i] : make_float2(0.f, 0.f);
     const float ix = iv.x, iy = iv.y, iz = 0.f, iw = 1.f;
     float o[4];
 #pragma unroll
@@ -154,9 +155,21 @@ __global__ void k_transform(uint32_t V, const float2 *__restrict__ vtx, const ui
     case 0x21: bits = 1u << 6; break;
     default: bits = 1u << 7; break;
     }
-    uint32_t p = vpath[i];
-    if ((__hip_atomic_load(&pflags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bits) != bits)
-        atomicOr(&pflags[p], bits);
+    // OR the region bits into the path's flags: one atomic per distinct path of the wave
+    const int lane = threadIdx.x & 63;
+    uint64_t pend = __ballot(live);
+    while (pend) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const uint32_t lp = (uint32_t)__builtin_amdgcn_readlane((int)p, leader);
+        const bool mine = ((pend >> lane) & 1ull) && p == lp;
+        const uint64_t grp = __ballot(mine);
+        uint32_t acc = mine ? bits : 0u;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) acc |= (uint32_t)__shfl_xor((int)acc, d, 64);
+        if (lane == leader) atomicOr(&pflags[lp], acc);
+        pend &= ~grp;
+    }
+    if (!live) return;
     if (floorf(ox) == ox) ox = ox - 1.f / 256.f;
     out[i] = make_float2(ox, oy);
 }

@@ -79,15 +79,24 @@ struct Stencil {
                 acc[k][b] = neg ? acc[k][b] - a : acc[k][b] + a;
             }
     }
+    // per byte lane: v = (0x80 + counter) & 0x7f; nonzero keeps v != 0, even-odd keeps
+    // bit 0.  v <= 0x7f, so v + 0x7f carries into bit 7 of its own byte exactly when v != 0.
     __device__ __forceinline__ void finish(int rule, uint32_t *out) const {
 #pragma unroll
         for (int k = 0; k < NW; ++k) {
             uint32_t ret = 0u;
+            if (rule == 0) {
 #pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                uint32_t v = (0x80808080u + acc[k][b]) & 0x7f7f7f7fu;
-                if (rule == 0) { v |= v >> 4; v |= v >> 2; v |= v >> 1; }
-                ret |= (v & 0x01010101u) << b;
+                for (int b = 0; b < 8; ++b) {
+                    const uint32_t v = (0x80808080u + acc[k][b]) & 0x7f7f7f7fu;
+                    ret |= ((v + 0x7f7f7f7fu) & 0x80808080u) >> (7 - b);
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const uint32_t v = (0x80808080u + acc[k][b]) & 0x7f7f7f7fu;
+                    ret |= (v & 0x01010101u) << b;
+                }
             }
             out[k] = ret;
         }
@@ -965,10 +974,11 @@ __device__ __forceinline__ float unorm_dec(int k) {
 }
 
 __device__ __forceinline__ int srgb_k0(float v) {
-    const float e = v <= 0.0031308f
-                        ? v * (12.92f * 255.f)
-                        : __builtin_fmaf(__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(v) * (1.f / 2.4f)),
-                                         1.055f * 255.f, -0.055f * 255.f);
+    // both branches evaluated and selected (no divergent branch per channel)
+    const float e_lin = v * (12.92f * 255.f);
+    const float e_pow = __builtin_fmaf(__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(v) * (1.f / 2.4f)),
+                                       1.055f * 255.f, -0.055f * 255.f);
+    const float e = v <= 0.0031308f ? e_lin : e_pow;
     return tmin(tmax((int)(e + 0.25f), 0), 255);
 }
 

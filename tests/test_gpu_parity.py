@@ -154,3 +154,31 @@ def test_bucket_size_classes(rasterizer, length):
     vg.addPolygon([(9.1, 12.3), (9.1 + length / 3, 12.5), (9.1 + length / 4, 13.7)])
     for spp in (8, 32):
         compare(rasterizer, vg, 2048, 24, spp, label="sliver%d" % length)
+
+
+def test_text_like_scene_reduced(rasterizer):
+    """The text-like workload's construction (4096x4096 canvas, glyph contours of quadratics
+    with reversed holes, nonzero, 16 spp) with 2% of its glyphs: every stage bit-exact."""
+    scene, W, H, spp = scanline.config_scene(3, scale=0.02)
+    cnt, exact = compare(rasterizer, scene, W, H, spp, label="config3-2%")
+    assert cnt["curve_fragment"] > 100000
+
+
+def test_text_like_scene_full_image(rasterizer):
+    """The full text-like workload (100,000 glyphs at 4096x4096, 16 spp): counts and the
+    whole image against the OpenMP build of the oracle."""
+    scene, W, H, spp = scanline.config_scene(3)
+    print("parity case: config3 full", W, H, spp, flush=True)
+    ref = pyoracle.run(scene, W, H, spp, srgb=True, ms=False, keep=False, omp=True)
+    rasterizer.setSamples(spp)
+    rasterizer.enableSRGBCorrection(True)
+    rasterizer.setMultisampleOutput(False)
+    rasterizer.resizeOutputBuffer(W, H)
+    rasterizer.setInputTransform(np.eye(4, dtype=np.float32))
+    rasterizer.dumpDebugData(False)
+    rasterizer.addVg(scene)
+    rasterizer.rasterize()
+    cnt = rasterizer.elementNumber()
+    for k, v in ref.counts().items():
+        assert cnt[k] == v, (k, cnt[k], v)
+    np.testing.assert_array_equal(rasterizer.output(), ref.get("image").reshape(H, W, 4))
