@@ -74,7 +74,7 @@ struct sl_handle {
     DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
         b_boff, b_bwsum, b_bcarry, b_bcursor, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
         b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
-        b_rowbeg, b_rowend, b_rrec, b_image, b_scan_tmp;
+        b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_image, b_scan_tmp;
     // debug host copies
     std::vector<int32_t> d_ts, d_yx, d_pid, d_draw;
     std::vector<int8_t> d_wn;
@@ -364,6 +364,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         (rc = ensure(h, h->b_nzv1, (size_t)NB * 4)) || (rc = ensure(h, h->b_rs_hist, 256 * rs_tiles * 4 + 16)) ||
         (rc = ensure(h, h->b_rs_hoff, (256 * rs_tiles + 1) * 4 + 16)) || (rc = ensure(h, h->b_rowbeg, R * 4)) ||
         (rc = ensure(h, h->b_rowend, R * 4)) || (rc = ensure(h, h->b_rrec, (size_t)NB * 16)) ||
+        (rc = ensure(h, h->b_btile, (size_t)NB * 8)) || (rc = ensure(h, h->b_rtile, (size_t)NB * 8)) ||
         (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
         return rc;
     f.binfo = P_<int2>(h->b_binfo); f.bcount = P_<uint32_t>(h->b_bcount); f.boff = P_<uint32_t>(h->b_boff);
@@ -378,6 +379,8 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     f.rs_hist = P_<uint32_t>(h->b_rs_hist); f.rs_hoff = P_<uint32_t>(h->b_rs_hoff);
     f.rowbeg = P_<uint32_t>(h->b_rowbeg); f.rowend = P_<uint32_t>(h->b_rowend);
     f.rrec = P_<uint4>(h->b_rrec);
+    f.btile = P_<uint64_t>(h->b_btile);
+    f.rtile = P_<uint64_t>(h->b_rtile);
     f.scan_tmp = P_<uint32_t>(h->b_scan_tmp);
 
     sl::launch_bucket_info(sc, f, s);
@@ -527,7 +530,7 @@ int sl_destroy(sl_handle *h) {
                       &h->b_rec_mask, &h->b_scratch, &h->b_items, &h->b_imask, &h->b_icov, &h->b_bicnt, &h->b_bmcnt, &h->b_bext,
                       &h->b_big_list, &h->b_small, &h->b_nz_flag, &h->b_nz_off, &h->b_nzk0, &h->b_nzk1, &h->b_nzv0,
                       &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp,
-                      &h->b_rrec, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
+                      &h->b_rrec, &h->b_btile, &h->b_rtile, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
     for (DevBuf *b : bufs) release(*b);
     if (h->events_ready)
         for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(h->ev[i]);

@@ -88,6 +88,7 @@ struct Frame {
     uint32_t *bicnt;         // [NB+1]
     uint32_t *bmcnt;         // [NB+1]
     uint32_t *bext;          // [NB]
+    uint64_t *btile;         // [NB] 64-pixel tiles of the row holding items
     uint32_t *big_list;      // [NB] buckets with more than 64 fragments
     uint32_t *big_count;     // [1]
     uint32_t *err_flag;      // [1]
@@ -112,6 +113,7 @@ struct Frame {
     uint32_t *rowbeg;        // [R]
     uint32_t *rowend;        // [R]
     uint4 *rrec;             // [NB] row-list records {extent, colour, items, item base}
+    uint64_t *rtile;         // [NB] row-list tile masks
     uint8_t *image;          // [H * pitch]
     int pitch;
     int srgb, ms;

@@ -149,6 +149,7 @@ struct ResolveArgs {
     uint32_t *bicnt;
     uint32_t *bmcnt;
     uint32_t *bext;
+    uint64_t *btile;         // [NB] 64-pixel tiles of the row holding items (W <= 4096)
     uint32_t *big_list;
     uint32_t *big_count;
     uint32_t *err;
@@ -168,6 +169,21 @@ __device__ __forceinline__ int dpp_scan(int x, int ident, Op op) {
 struct OpAdd { __device__ int operator()(int a, int b) const { return a + b; } };
 struct OpMin { __device__ int operator()(int a, int b) const { return a < b ? a : b; } };
 struct OpMax { __device__ int operator()(int a, int b) const { return a > b ? a : b; } };
+struct OpOr { __device__ int operator()(int a, int b) const { return a | b; } };
+
+// 64-pixel tiles spanned by [lo, hi) as bits of a 64-bit mask (all tiles when the canvas
+// is wider than 64 tiles), OR-reduced over the wave into lane 63.
+__device__ __forceinline__ uint64_t tile_bits(int lo, int hi, int W) {
+    if (lo >= hi) return 0ull;
+    if (W > 4096) return ~0ull;
+    const int t0 = lo >> 6, t1 = (hi - 1) >> 6;
+    return (~0ull << t0) & (~0ull >> (63 - t1));
+}
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(dpp_scan((int)(uint32_t)v, 0, OpOr()), 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(dpp_scan((int)(uint32_t)(v >> 32), 0, OpOr()), 63);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
 __device__ __forceinline__ int scan_add(int x) { return dpp_scan(x, 0, OpAdd()); }
 __device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, 63); }
 
@@ -228,6 +244,7 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
         wcarry -= A.scarry[cv.w];
     }
     int icarry = 0, nmerged = 0, xmin = INT_MAX, xmax = INT_MIN;
+    uint64_t tmask = 0ull;
     const size_t ib = 2 * (size_t)base;
 #pragma unroll
     for (int c = 0; c < S; ++c) {
@@ -276,12 +293,14 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
         const int hi = merged ? x + 2 : (span ? x : INT_MIN);
         xmin = tmin(xmin, lane63(dpp_scan(lo, INT_MAX, OpMin())));
         xmax = tmax(xmax, lane63(dpp_scan(hi, INT_MIN, OpMax())));
+        tmask |= wave_or64(tile_bits(lo, hi, W));
         nmerged += lane63(scan_add(merged ? 1 : 0));
     }
     if (lane == 0) {
         A.bicnt[b] = (uint32_t)icarry;
         A.bmcnt[b] = (uint32_t)nmerged;
         A.bext[b] = icarry ? ((uint32_t)xmin | ((uint32_t)xmax << 16)) : 0xFFFFu;
+        A.btile[b] = tmask;
     }
     __syncthreads();
 }
@@ -390,10 +409,12 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
     const int xmax = lane63(dpp_scan(hi, INT_MIN, OpMax()));
     const int nmerged = lane63(scan_add(merged ? 1 : 0));
     const int icnt = lane63(ic);
+    const uint64_t tmask = wave_or64(tile_bits(lo, hi, W));
     if (lane == 0) {
         A.bicnt[b] = (uint32_t)icnt;
         A.bmcnt[b] = (uint32_t)nmerged;
         A.bext[b] = icnt ? ((uint32_t)xmin | ((uint32_t)xmax << 16)) : 0xFFFFu;
+        A.btile[b] = tmask;
     }
     __syncthreads();
 }
@@ -431,7 +452,7 @@ __global__ void __launch_bounds__(64) k_resolve_first(ResolveArgs A) {
         if (b + G < A.NB) small_load<NW>(A, b + G, base1, end1, info1, in1);
         const int n = (int)(end0 - base0);
         if (n == 0) {
-            if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; }
+            if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; A.btile[b] = 0ull; }
         } else if (n > 64) {
             if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)b;
         } else {
@@ -471,7 +492,7 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
         const int n = (int)(A.boff[b + 1] - base);
         if (n <= 512) continue;
         if (n > 65536) {
-            if (lane == 0) { atomicOr(A.err, 2u); A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; }
+            if (lane == 0) { atomicOr(A.err, 2u); A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; A.btile[b] = 0ull; }
             continue;
         }
         int P2 = 1;
@@ -502,6 +523,7 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
             wcarry -= A.scarry[cv.w];
         }
         int icarry = 0, nmerged = 0, xmin = INT_MAX, xmax = INT_MIN;
+        uint64_t tmask = 0ull;
         const size_t ib = 2 * (size_t)base;
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int p = c0 + lane;
@@ -550,12 +572,14 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
             const int hi = merged ? x + 2 : (span ? x : INT_MIN);
             xmin = tmin(xmin, lane63(dpp_scan(lo, INT_MAX, OpMin())));
             xmax = tmax(xmax, lane63(dpp_scan(hi, INT_MIN, OpMax())));
+            tmask |= wave_or64(tile_bits(lo, hi, W));
             nmerged += lane63(scan_add(merged ? 1 : 0));
         }
         if (lane == 0) {
             A.bicnt[b] = (uint32_t)icarry;
             A.bmcnt[b] = (uint32_t)nmerged;
             A.bext[b] = icarry ? ((uint32_t)xmin | ((uint32_t)xmax << 16)) : 0xFFFFu;
+            A.btile[b] = tmask;
         }
         __syncthreads();
     }
@@ -1022,6 +1046,7 @@ struct CovArgs {
     int W, H, tiles_x;
     const uint32_t *rowbeg, *rowend;
     const uint4 *rrec;          // per row-list entry: extent, colour, item count, item base
+    const uint64_t *rtile;      // per row-list entry: 64-pixel tiles holding items
     const uint32_t *items, *icov;
     const float *dec_c;         // [256] colour decode, for the paint colours
     const float4 *step_c;       // [256] sRGB steps {thr[k+1], dec[k], dec[k+1], 0}
@@ -1162,14 +1187,20 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
     };
 
     const uint32_t q0 = A.rowbeg[row0 + r], q1 = A.rowend[row0 + r];
+    const int tbit = (tx0 >> 6) & 63;
     int nlist = 0;
     uint4 nxt = (q0 + lane < q1) ? A.rrec[q0 + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
+    uint64_t nxt_t = (q0 + lane < q1) ? A.rtile[q0 + lane] : 0ull;
     for (uint32_t qc = q0; qc < q1; qc += 64) {
         const uint4 rec = nxt;
+        const uint64_t rt = nxt_t;
         const bool inq = qc + lane < q1;
-        if (qc + 64 < q1) nxt = (qc + 64 + lane < q1) ? A.rrec[qc + 64 + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
+        if (qc + 64 < q1) {
+            nxt = (qc + 64 + lane < q1) ? A.rrec[qc + 64 + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
+            nxt_t = (qc + 64 + lane < q1) ? A.rtile[qc + 64 + lane] : 0ull;
+        }
         const int exlo = (int)(rec.x & 0xFFFFu), exhi = (int)(rec.x >> 16);
-        const bool ov = inq && exlo < tx1 && exhi > tx0;
+        const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tbit) & 1ull);
         const unsigned long long om = __ballot(ov);
         if (!om) continue;
         const int nov = (int)__popcll(om);
@@ -1198,11 +1229,13 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
 __global__ void k_row_records(int64_t NB, const uint32_t *__restrict__ n_ptr, const uint32_t *__restrict__ rowvals,
                               const uint32_t *__restrict__ bext, const int2 *__restrict__ binfo,
                               const uint32_t *__restrict__ finfo, const uint32_t *__restrict__ bicnt,
-                              const uint32_t *__restrict__ boff, uint4 *__restrict__ rrec) {
+                              const uint32_t *__restrict__ boff, const uint64_t *__restrict__ btile,
+                              uint4 *__restrict__ rrec, uint64_t *__restrict__ rtile) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= NB || q >= (int64_t)*n_ptr) return;
     const uint32_t bb = rowvals[q];
     rrec[q] = make_uint4(bext[bb], finfo[binfo[bb].x], bicnt[bb], 2u * boff[bb]);
+    rtile[q] = btile[bb];
 }
 
 // Device requantisation of arbitrary values (colour-table self-check): codes and the
@@ -1314,6 +1347,7 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
     A.rec_meta = f.rec_meta; A.rec_mask = f.rec_mask; A.scratch = f.scratch_keys; A.W = f.W;
     A.pcanvas = sc.pcanvas; A.scarry = f.scarry;
     A.items = f.items; A.imask = f.imask; A.icov = f.icov; A.bicnt = f.bicnt; A.bmcnt = f.bmcnt; A.bext = f.bext;
+    A.btile = f.btile;
     A.big_list = f.big_list; A.big_count = f.big_count; A.err = f.err_flag;
     launch_fill_u32(f.big_count, 0u, 1, s);
     const unsigned g1 = (unsigned)std::min<int64_t>(f.NB, f.resolve_grid > 0 ? f.resolve_grid : kResolveBlocks);
@@ -1384,10 +1418,10 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
             hipLaunchKernelGGL(k_row_records, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB,
                                (const uint32_t *)(f.nz_off + f.NB), (const uint32_t *)f.nz_vals[0],
                                (const uint32_t *)f.bext, (const int2 *)f.binfo, sc.finfo, (const uint32_t *)f.bicnt,
-                               (const uint32_t *)f.boff, f.rrec);
+                               (const uint32_t *)f.boff, (const uint64_t *)f.btile, f.rrec, f.rtile);
         CovArgs C;
         C.W = f.W; C.H = f.H; C.tiles_x = tiles_x;
-        C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec;
+        C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec; C.rtile = f.rtile;
         C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.step_c = t.step_c;
         C.image = f.image; C.pitch = f.pitch;
         C.nscenes = f.nscenes; C.ntiles = ntiles; C.stile = f.stile; C.simg = f.simg;
