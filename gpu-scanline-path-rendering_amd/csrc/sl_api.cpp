@@ -350,7 +350,8 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     if (!replay) h->NB = NB;
     const size_t R = (size_t)f.R;
     const size_t rs_tiles = ((size_t)NB + 2047) / 2048;
-    scan_need = sl::scan_tmp_elems_needed(std::max<size_t>({(size_t)C + 1, (size_t)P + 1, (size_t)NB + 1, 256 * rs_tiles + 1}));
+    scan_need = sl::scan_tmp_elems_needed(
+        std::max<size_t>({(size_t)C + 1, (size_t)P + 1, (size_t)NB + 1, 256 * rs_tiles + 1, (size_t)F + 1}));
     if ((rc = ensure(h, h->b_binfo, (size_t)NB * 8)) || (rc = ensure(h, h->b_bcount, ((size_t)NB + 1) * 4)) ||
         (rc = ensure(h, h->b_boff, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bwsum, ((size_t)NB + 1) * 4)) ||
         (rc = ensure(h, h->b_bcarry, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bcursor, (size_t)NB * 4)) ||
@@ -372,6 +373,11 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     f.rec_meta = P_<uint32_t>(h->b_rec_meta); f.rec_mask = P_<uint32_t>(h->b_rec_mask);
     f.scratch_keys = P_<uint32_t>(h->b_scratch); f.items = P_<uint32_t>(h->b_items); f.imask = P_<uint32_t>(h->b_imask);
     f.icov = P_<uint32_t>(h->b_icov);
+    // the rank array lives in the large-bucket scratch (used only later, by the resolve);
+    // the winding scan's input and output live in the item arrays the resolve overwrites
+    f.frank = f.scratch_keys;
+    f.rec_swn = f.items;
+    f.wscan = f.imask;
     f.bicnt = P_<uint32_t>(h->b_bicnt); f.bmcnt = P_<uint32_t>(h->b_bmcnt); f.bext = P_<uint32_t>(h->b_bext);
     f.big_list = P_<uint32_t>(h->b_big_list); f.nz_flag = P_<uint32_t>(h->b_nz_flag); f.nz_off = P_<uint32_t>(h->b_nz_off);
     f.nz_keys[0] = P_<uint32_t>(h->b_nzk0); f.nz_keys[1] = P_<uint32_t>(h->b_nzk1);
@@ -388,9 +394,9 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     mark(4);
     sl::launch_frag_count(sc, tb, f, s);
     sl::scan_exclusive_u32(f.bcount, f.boff, NB, f.scan_tmp, s);
-    sl::scan_exclusive_u32(f.bwsum, f.bcarry, NB, f.scan_tmp, s);
-    sl::launch_scene_carry(sc, f, s);
     sl::launch_frag_scatter(sc, tb, f, s);
+    sl::launch_bucket_carry(f, s);
+    sl::launch_scene_carry(sc, f, s);
     if ((rc = check_launch(h, "fragments"))) return rc;
     mark(5);
     sl::launch_resolve(sc, tb, f, s);

@@ -333,11 +333,16 @@ __device__ __forceinline__ void load_fragments(const SceneDev &sc, const float2 
     }
 }
 
+// Count pass: every fragment that lands in a bucket (a valid row, or an out-of-canvas row
+// whose scan winding still feeds the winding carry) takes its rank in the bucket from the
+// bucket counter: one returning atomic per distinct bucket of the wave, the lanes of a
+// bucket numbered by lane order.  The rank lets the scatter pass place records without
+// atomics.
 template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t *__restrict__ tab,
                                                      const float2 *__restrict__ tv, const int2 *__restrict__ ts, int64_t F,
                                                      int W, int H, const int2 *__restrict__ prow,
-                                                     const uint32_t *__restrict__ pbase, uint32_t *bcount, uint32_t *bwsum,
+                                                     const uint32_t *__restrict__ pbase, uint32_t *bcount, uint32_t *rank,
                                                      uint32_t *err) {
     constexpr int K = kFragPerThread;
     const int lane = threadIdx.x & 63;
@@ -349,45 +354,45 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
     load_fragments<K>(sc, tv, ts, F, i0, prow, pbase, in, live, pr, pb);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        bool act = live[j];
+        bool act = false;
         uint32_t b = 0;
-        int row = -1, swn = 0;
-        if (act) {
+        if (live[j]) {
             FragMaskIn mi;
             const Frag f = frag_geometry<SPP, false>(sc, in[j], W, H, mi);
             bool ok;
             b = (uint32_t)bucket_of(pr[j], pb[j], f.row, ok);
             if (!ok) atomicOr(err, 1u);
-            act = ok;
-            row = f.row;
-            swn = f.swn;
+            act = ok && (f.row >= 0 || f.swn != 0);
         }
-        // lanes of the wave with the same bucket share one atomic (the wave walks a few
-        // curves, so it touches few buckets)
-        uint64_t pend = __ballot(act && (row >= 0 || swn != 0));
+        uint64_t pend = __ballot(act);
+        uint32_t base = 0, r = 0;
+        int my_leader = 0;
         while (pend) {
             const int leader = __ffsll((long long)pend) - 1;
             const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, leader);
             const bool mine = ((pend >> lane) & 1ull) && b == lb;
             const uint64_t grp = __ballot(mine);
-            const uint32_t nc = (uint32_t)__popcll(__ballot(mine && row >= 0));
-            const int nw = (int)__popcll(__ballot(mine && swn > 0)) - (int)__popcll(__ballot(mine && swn < 0));
-            if (lane == leader) {
-                if (nc) atomicAdd(&bcount[lb], nc);
-                if (nw) atomicAdd(&bwsum[lb], (uint32_t)nw);
+            if (mine) {
+                my_leader = leader;
+                r = (uint32_t)__popcll(grp & lanes_lt());
             }
+            if (lane == leader) base = atomicAdd(&bcount[lb], (uint32_t)__popcll(grp));
             pend &= ~grp;
         }
+        base = (uint32_t)__shfl((int)base, my_leader, 64);
+        if (live[j]) rank[i0 + 64 * j] = act ? base + r : 0xFFFFFFFFu;
     }
 }
 
+// Scatter pass: records at boff[bucket] + rank, and each record's scan winding (sign
+// extended) at the same slot for the winding-carry scan.
 template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_t *__restrict__ tab,
                                                        const float2 *__restrict__ tv, const int2 *__restrict__ ts,
                                                        int64_t F, int W, int H, const int2 *__restrict__ prow,
                                                        const uint32_t *__restrict__ pbase,
-                                                       const uint32_t *__restrict__ boff, uint32_t *bcursor,
-                                                       uint32_t *rec_meta, uint32_t *rec_mask) {
+                                                       const uint32_t *__restrict__ boff, const uint32_t *__restrict__ rank,
+                                                       uint32_t *rec_meta, uint32_t *rec_mask, uint32_t *rec_swn) {
     constexpr int NW = Words<SPP>::n;
     constexpr int K = kFragPerThread;
     const int lane = threadIdx.x & 63;
@@ -397,50 +402,34 @@ __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_
     int2 pr[K];
     uint32_t pb[K];
     load_fragments<K>(sc, tv, ts, F, i0, prow, pbase, in, live, pr, pb);
+    uint32_t rk[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) rk[j] = live[j] ? rank[i0 + 64 * j] : 0xFFFFFFFFu;
     Frag f[K];
     FragMaskIn mi[K];
-    bool act[K];
     uint32_t b[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        act[j] = false;
         b[j] = 0;
-        if (live[j]) {
+        if (rk[j] != 0xFFFFFFFFu) {
             f[j] = frag_geometry<SPP, true>(sc, in[j], W, H, mi[j]);
             bool ok;
             b[j] = (uint32_t)bucket_of(pr[j], pb[j], f[j].row, ok);
-            act[j] = ok && f[j].row >= 0;
+            if (f[j].row < 0) mi[j].live = false;  // winding only: no mask needed
         }
     }
 #pragma unroll
     for (int j = 0; j < K; ++j)
-        if (act[j]) frag_mask<SPP>(tab, mi[j], f[j]);
+        if (rk[j] != 0xFFFFFFFFu) frag_mask<SPP>(tab, mi[j], f[j]);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        // one cursor atomic per distinct bucket of the wave; each leader keeps the returned
-        // base in its own lane, and every lane reads its leader's base afterwards
-        uint64_t pend = __ballot(act[j]);
-        uint32_t base = 0, rank = 0;
-        int my_leader = 0;
-        while (pend) {
-            const int leader = __ffsll((long long)pend) - 1;
-            const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b[j], leader);
-            const bool mine = ((pend >> lane) & 1ull) && b[j] == lb;
-            const uint64_t grp = __ballot(mine);
-            if (mine) {
-                my_leader = leader;
-                rank = (uint32_t)__popcll(grp & lanes_lt());
-            }
-            if (lane == leader) base = atomicAdd(&bcursor[lb], (uint32_t)__popcll(grp));
-            pend &= ~grp;
-        }
-        base = (uint32_t)__shfl((int)base, my_leader, 64);
-        if (!act[j]) continue;
-        const uint32_t slot = boff[b[j]] + base + rank;
+        if (rk[j] == 0xFFFFFFFFu) continue;
+        const uint32_t slot = boff[b[j]] + rk[j];
         if (slot >= boff[b[j] + 1]) continue;
         const Frag &g = f[j];
         const uint32_t cx = (uint32_t)((g.pos_x + 2) >> 1);
         rec_meta[slot] = cx | (((uint32_t)g.swn & 0xFFu) << 16) | ((g.wnc < 0 ? 1u : 0u) << 24);
+        rec_swn[slot] = (uint32_t)g.swn;
         if (NW == 4) {
             *(uint4 *)(rec_mask + (size_t)slot * 4) = make_uint4(g.m[0], g.m[1], g.m[2], g.m[3]);
         } else if (NW == 2) {
@@ -449,6 +438,14 @@ __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_
             rec_mask[slot] = g.m[0];
         }
     }
+}
+
+// Winding carry of every bucket: the exclusive sum (mod 2^32) of the scan windings of all
+// records before it, bucket order being the reference's fragment order.
+__global__ void k_bucket_carry(int64_t NB, const uint32_t *__restrict__ boff, const uint32_t *__restrict__ wscan,
+                               uint32_t *bcarry) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b <= NB) bcarry[b] = wscan[boff[b]];
 }
 
 template <int SPP>
@@ -486,7 +483,7 @@ void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_
     dim3 g(blocks_for(f.F, fb * kFragPerThread)), b(fb);
 #define SL_LAUNCH(N)                                                                                              \
     hipLaunchKernelGGL(k_frag_count<N>, g, b, 0, s, sc, t.pixel, (const float2 *)f.tvtx, (const int2 *)f.ts, f.F, \
-                       f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, f.bcount, f.bwsum, f.err_flag)
+                       f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, f.bcount, f.frank, f.err_flag)
     if (t.spp == 8) SL_LAUNCH(8);
     else if (t.spp == 16) SL_LAUNCH(16);
     else SL_LAUNCH(32);
@@ -500,11 +497,18 @@ void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStrea
 #define SL_LAUNCH(N)                                                                                                \
     hipLaunchKernelGGL(k_frag_scatter<N>, g, b, 0, s, sc, t.pixel, (const float2 *)f.tvtx, (const int2 *)f.ts, f.F, \
                        f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, (const uint32_t *)f.boff,         \
-                       f.bcursor, f.rec_meta, f.rec_mask)
+                       (const uint32_t *)f.frank, f.rec_meta, f.rec_mask, f.rec_swn)
     if (t.spp == 8) SL_LAUNCH(8);
     else if (t.spp == 16) SL_LAUNCH(16);
     else SL_LAUNCH(32);
 #undef SL_LAUNCH
+}
+
+void launch_bucket_carry(Frame &f, hipStream_t s) {
+    // records' scan windings -> exclusive scan -> carry at each bucket start
+    scan_exclusive_u32(f.rec_swn, f.wscan, (size_t)f.F, f.scan_tmp, s);
+    hipLaunchKernelGGL(k_bucket_carry, dim3(blocks_for(f.NB + 1, 256)), dim3(256), 0, s, f.NB, (const uint32_t *)f.boff,
+                       (const uint32_t *)f.wscan, f.bcarry);
 }
 
 void launch_debug_fragments(const SceneDev &sc, const Tables &t, Frame &f, int32_t *yx, int32_t *pid, int8_t *wn,

@@ -78,6 +78,9 @@ struct Frame {
     uint32_t *bwsum;         // [NB+1] -> bcarry
     uint32_t *bcarry;        // [NB+1]
     uint32_t *bcursor;       // [NB]
+    uint32_t *frank;         // [F] rank of each fragment in its bucket (~0: none)
+    uint32_t *rec_swn;       // [F] scan winding of each record, sign-extended
+    uint32_t *wscan;         // [F+1] exclusive scan of rec_swn
     uint32_t *rec_meta;      // [F] (valid fragments only are written)
     uint32_t *rec_mask;      // [F * words]
     uint32_t *scratch_keys;  // [F] sort keys of large buckets
@@ -134,6 +137,7 @@ void launch_path_buckets(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_bucket_info(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
 void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
+void launch_bucket_carry(Frame &f, hipStream_t s);
 void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s);
 void launch_scene_carry(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s);

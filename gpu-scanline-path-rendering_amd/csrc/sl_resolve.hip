@@ -450,7 +450,8 @@ __global__ void __launch_bounds__(64) k_resolve_first(ResolveArgs A) {
         int2 info2;
         head(b + 2 * G, base2, end2, info2);
         if (b + G < A.NB) small_load<NW>(A, b + G, base1, end1, info1, in1);
-        const int n = (int)(end0 - base0);
+        // the out-of-canvas bucket holds records only for the winding carry
+        const int n = info0.y < 0 ? 0 : (int)(end0 - base0);
         if (n == 0) {
             if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; A.btile[b] = 0ull; }
         } else if (n > 64) {
