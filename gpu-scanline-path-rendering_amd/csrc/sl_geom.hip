@@ -374,6 +374,19 @@ __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__r
 // walks of long chains start early, and each wave's lanes carry similar lengths.
 __device__ __forceinline__ int len_class(uint32_t L) { return L >= 128 ? 0 : (L >= 16 ? 1 : 2); }
 
+__device__ __forceinline__ uint32_t wave_excl_scan_dpp(uint32_t v, uint32_t &total) {
+    // inclusive DPP scan (row_shr 1,2,4,8 + row_bcast 15/31), then exclusive by subtracting v
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    total = (uint32_t)__builtin_amdgcn_readlane(x, 63);
+    return (uint32_t)x - v;
+}
+
 __global__ void __launch_bounds__(256) k_chain_list(uint32_t C, const uint32_t *__restrict__ nchain,
                                                     const uint32_t *__restrict__ pcnt, uint32_t cstride,
                                                     uint32_t pstride, uint32_t *clist, uint32_t *plist,
@@ -382,11 +395,14 @@ __global__ void __launch_bounds__(256) k_chain_list(uint32_t C, const uint32_t *
     const int lane = threadIdx.x & 63;
     const bool live = c < C;
     const int np = live ? (int)(nchain[c] >> 1) : 0;
-    uint32_t pc[5] = {0u, 0u, 0u, 0u, 0u};
-    for (int k = 0; k < np; ++k) pc[k] = pcnt[5 * (size_t)c + k];
+    uint32_t pc[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pc[k] = k < np ? pcnt[5 * (size_t)c + k] : 0u;
     // per-thread counts per class, one reservation per class and wave
     uint32_t n[6] = {0u, 0u, 0u, 0u, 0u, 0u};
-    for (int k = 0; k < np; ++k) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (k >= np) break;
         const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
         if (nx) n[len_class(nx)]++;
         if (ny) n[len_class(ny)]++;
@@ -395,19 +411,15 @@ __global__ void __launch_bounds__(256) k_chain_list(uint32_t C, const uint32_t *
     uint32_t off[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        uint32_t inc = n[q];
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += o;
-        }
-        const uint32_t tot = (uint32_t)__shfl((int)inc, 63, 64);
+        uint32_t tot;
+        const uint32_t ex = wave_excl_scan_dpp(n[q], tot);
         uint32_t base = 0;
-        if (lane == 63 && tot) base = atomicAdd(&counts[q], tot);
-        base = (uint32_t)__shfl((int)base, 63, 64);
-        off[q] = base + inc - n[q];
+        if (lane == 0 && tot) base = atomicAdd(&counts[q], tot);
+        off[q] = (uint32_t)__builtin_amdgcn_readlane((int)base, 0) + ex;
     }
-    for (int k = 0; k < np; ++k) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (k >= np) break;
         const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
         if (nx) {
             const int q = len_class(nx);
@@ -424,8 +436,9 @@ __global__ void __launch_bounds__(256) k_chain_list(uint32_t C, const uint32_t *
 
 // entry g of the class lists, in class order; false past their end
 __device__ __forceinline__ bool class_entry(uint32_t g, const uint32_t *__restrict__ list,
-                                            const uint32_t *__restrict__ cnt, uint32_t stride, uint32_t &d) {
-    const uint32_t n0 = cnt[0], n1 = cnt[1], n2 = cnt[2];
+                                            const uint32_t *__restrict__ cnt, uint32_t stride, uint32_t &d,
+                                            bool skip_first = false) {
+    const uint32_t n0 = skip_first ? 0u : cnt[0], n1 = cnt[1], n2 = cnt[2];
     if (g < n0) { d = list[g]; return true; }
     g -= n0;
     if (g < n1) { d = list[stride + g]; return true; }
@@ -517,38 +530,44 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
     }
 }
 
-// Intersection pass 2: one thread per monotone piece replays the reference's slot walk
-// (ras_cut.h:910-1000) over the piece's solved chains: pick the x slot when x <= y, emit
-// its t, refill it from the chain or with the exhausted marker 2 | side; equal consecutive
-// t's OR their flags into the previous entry.  A piece starts at the previous piece's end
+// Intersection pass 2 replays the reference's slot walk (ras_cut.h:910-1000) per
+// monotone piece over the piece's solved chains: pick the x slot when x <= y, emit its t,
+// refill it from the chain or with the exhausted marker 2 | side; equal consecutive t's
+// OR their flags into the previous entry.  A piece starts at the previous piece's end
 // flags (t = 0 for the first), exactly as the per-curve walk carries them.
-__global__ void __launch_bounds__(256) k_merge(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
-                                                const uint8_t *__restrict__ cncut, const uint32_t *__restrict__ coff,
-                                                const uint32_t *__restrict__ pcnt, const float *__restrict__ chain_t,
-                                                const uint32_t *__restrict__ plist, const uint32_t *__restrict__ counts,
-                                                uint32_t pstride, int2 *ts) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t d;
-    if (!class_entry(g, plist, counts, pstride, d)) return;
-    const uint32_t c = d >> 3;
+struct PieceRef {
+    uint32_t c, base, nx, ny;
+    float t0m;
+};
+
+__device__ __forceinline__ PieceRef piece_ref(const SceneDev &sc, const float2 *__restrict__ tv,
+                                              const float4 *__restrict__ ccuts, const uint32_t *__restrict__ coff,
+                                              const uint32_t *__restrict__ pcnt, uint32_t d) {
+    PieceRef r;
+    r.c = d >> 3;
     const int k = (int)(d & 7u);
-    const Curve cu = load_curve(sc, tv, c);
-    const float4 q = ccuts[c];
-    const int n = cncut[c];
-    const uint32_t *pc = pcnt + 5 * (size_t)c;
+    const uint32_t *pc = pcnt + 5 * (size_t)r.c;
     uint32_t p = 0;
     for (int kk = 0; kk < k; ++kk) p += 1u + (pc[kk] & 0xFFFFu) + (pc[kk] >> 16);
-    const uint32_t base = coff[c] + p;
-    int2 *out = ts + base;
-    const float *X = chain_t + base;
-    const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
-    const float *Y = X + nx;
-    float t0m = 0.f;
+    r.base = coff[r.c] + p;
+    r.nx = pc[k] & 0xFFFFu;
+    r.ny = pc[k] >> 16;
+    r.t0m = 0.f;
     if (k > 0) {
+        const Curve cu = load_curve(sc, tv, r.c);
+        const float4 q = ccuts[r.c];
         const float tr = pick5(k - 1, q.x, q.y, q.z, q.w, 1.f);
-        t0m = piece_end_flags(tr, ex(cu, tr));
+        r.t0m = piece_end_flags(tr, ex(cu, tr));
     }
-    (void)n;
+    return r;
+}
+
+__device__ __forceinline__ void walk_piece(const PieceRef &r, const float *__restrict__ chain_t, int2 *ts) {
+    int2 *out = ts + r.base;
+    const float *X = chain_t + r.base;
+    const uint32_t nx = r.nx, ny = r.ny;
+    const float *Y = X + nx;
+    const float t0m = r.t0m;
     const float two = 2.f, two1 = bf(fb(2.f) | 1u);
     float xs, ys;
     uint32_t xi = 0, yi = 0, rx, ry;
@@ -571,8 +590,87 @@ __global__ void __launch_bounds__(256) k_merge(SceneDev sc, const float2 *__rest
             o |= last;
             out[it - 1].y = (int)o;
         }
-        out[it] = make_int2((int)c, (int)o);
+        out[it] = make_int2((int)r.c, (int)o);
         last = o;
+    }
+}
+
+// Short and medium pieces: one thread per piece walks it.
+__global__ void __launch_bounds__(256) k_merge(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
+                                                const uint32_t *__restrict__ coff, const uint32_t *__restrict__ pcnt,
+                                                const float *__restrict__ chain_t, const uint32_t *__restrict__ plist,
+                                                const uint32_t *__restrict__ counts, uint32_t pstride, int2 *ts) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t d;
+    if (!class_entry(g, plist, counts, pstride, d, true)) return;
+    walk_piece(piece_ref(sc, tv, ccuts, coff, pcnt, d), chain_t, ts);
+}
+
+// Long pieces: one wavefront per piece merges in parallel.  The x chain X and the y
+// stream B = [piece start, Y...] are each sorted except that B's start may exceed Y[0]
+// by flag bits alone; with B ordered by its running maximum the walk is a stable merge
+// (x first on ties): X[i] lands at i + #{B keys < X[i]}, B[j] at j + #{X <= key(B[j])}.
+// Runs of equal t then take the OR of their flags up to the next entry of the run.
+constexpr int kMergeCap = 2048;
+
+__global__ void __launch_bounds__(64) k_merge_long(SceneDev sc, const float2 *__restrict__ tv,
+                                                   const float4 *__restrict__ ccuts, const uint32_t *__restrict__ coff,
+                                                   const uint32_t *__restrict__ pcnt, const float *__restrict__ chain_t,
+                                                   const uint32_t *__restrict__ plist, const uint32_t *__restrict__ counts,
+                                                   int2 *ts) {
+    __shared__ float sa[kMergeCap], sb[kMergeCap];
+    __shared__ uint32_t sm[kMergeCap];
+    const int lane = threadIdx.x;
+    const uint32_t n0 = counts[0];
+    for (uint32_t g = blockIdx.x; g < n0; g += gridDim.x) {
+        const PieceRef r = piece_ref(sc, tv, ccuts, coff, pcnt, plist[g]);
+        const uint32_t nx = r.nx, ny = r.ny, n = nx + ny + 1;
+        if (n > (uint32_t)kMergeCap) {
+            // an over-long piece: the walk itself
+            if (lane == 0) walk_piece(r, chain_t, ts);
+            continue;
+        }
+        const float *X = chain_t + r.base;
+        const float *Y = X + nx;
+        if (nx == 0 || ny == 0) {
+            // one stream: the piece start, then that chain in order
+            const float *Z = nx == 0 ? Y : X;
+            for (uint32_t p = lane; p < n; p += 64) sm[p] = p == 0 ? fb(r.t0m) : fb(Z[p - 1]);
+        } else {
+        for (uint32_t i = lane; i < nx; i += 64) sa[i] = X[i];
+        for (uint32_t j = lane; j <= ny; j += 64) sb[j] = j == 0 ? r.t0m : Y[j - 1];
+        __syncthreads();
+        // running-maximum key of B[j]: max(t0m, Y[j-1]) for j >= 1 (Y is sorted)
+        auto bkey = [&](uint32_t j) { return j == 0 ? sb[0] : fmaxf(sb[0], sb[j]); };
+        for (uint32_t i = lane; i < nx; i += 64) {
+            const float x = sa[i];
+            uint32_t lo = 0, hi = ny + 1;  // #{j : key(B[j]) < x}
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (bkey(mid) < x) lo = mid + 1; else hi = mid;
+            }
+            sm[i + lo] = fb(x);
+        }
+        for (uint32_t j = lane; j <= ny; j += 64) {
+            const float key = bkey(j);
+            uint32_t lo = 0, hi = nx;  // #{i : X[i] <= key}
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sa[mid] <= key) lo = mid + 1; else hi = mid;
+            }
+            sm[j + lo] = fb(sb[j]);
+        }
+        }
+        __syncthreads();
+        int2 *out = ts + r.base;
+        for (uint32_t p = lane; p < n; p += 64) {
+            const uint32_t v = sm[p], mk = v & 0xFFFFFFFCu;
+            uint32_t acc = v;
+            for (uint32_t q = p; q > 0 && (sm[q - 1] & 0xFFFFFFFCu) == mk; --q) acc |= sm[q - 1];
+            if (p + 1 < n && (sm[p + 1] & 0xFFFFFFFCu) == mk) acc |= sm[p + 1];
+            out[p] = make_int2((int)r.c, (int)acc);
+        }
+        __syncthreads();
     }
 }
 
@@ -630,11 +728,15 @@ void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s) {
                            (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
                            (const uint32_t *)f.pcnt, (const uint32_t *)f.chains, (const uint32_t *)f.lcount, cstride,
                            f.W, f.H, f.chain_t);
-    if (pstride)
+    if (pstride) {
+        hipLaunchKernelGGL(k_merge_long, dim3(4096), dim3(64), 0, s, sc, (const float2 *)f.tvtx, (const float4 *)f.ccuts,
+                           (const uint32_t *)f.coff, (const uint32_t *)f.pcnt, (const float *)f.chain_t,
+                           (const uint32_t *)f.plist, (const uint32_t *)(f.lcount + 3), f.ts);
         hipLaunchKernelGGL(k_merge, dim3(blocks_for(pstride, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                           (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
-                           (const uint32_t *)f.pcnt, (const float *)f.chain_t, (const uint32_t *)f.plist,
-                           (const uint32_t *)(f.lcount + 3), pstride, f.ts);
+                           (const float4 *)f.ccuts, (const uint32_t *)f.coff, (const uint32_t *)f.pcnt,
+                           (const float *)f.chain_t, (const uint32_t *)f.plist, (const uint32_t *)(f.lcount + 3),
+                           pstride, f.ts);
+    }
 }
 
 }  // namespace sl
