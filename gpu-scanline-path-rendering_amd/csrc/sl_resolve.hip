@@ -978,10 +978,9 @@ __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
 // again, exactly as an 8-bit framebuffer does:
 //   UNORM8: code = largest k with v >= (k - 0.5) / 255, i.e. with 510 v - (2k - 1) >= 0 --
 //           one fma gives that sign exactly; decode = RN(k / 255) by a corrected product.
-//   sRGB8:  an estimate k0 of the code from the transfer function (hardware log2/exp2;
-//           its error is far below the 0.25 code units the bracketing needs) leaves the
-//           code at k0 or k0 + 1; one 16-byte step entry {thr[k0+1], dec[k0], dec[k0+1]}
-//           decides and decodes.
+//   sRGB8:  a 4096-entry table gives the code k0 of the interval start floor(4096 v)/4096;
+//           no interval holds two thresholds, so the code is k0 or k0 + 1, and one 16-byte
+//           step entry {thr[k0+1], dec[k0], dec[k0+1]} decides and decodes.
 // ---------------------------------------------------------------------------------
 __device__ __forceinline__ int unorm_code(float v) {
     int k = tmin(tmax((int)__builtin_fmaf(v, 255.f, 0.5f), 0), 255);
@@ -998,17 +997,11 @@ __device__ __forceinline__ float unorm_dec(int k) {
     return __builtin_fmaf(__builtin_fmaf(-q, 255.f, fk), r, q);
 }
 
-__device__ __forceinline__ int srgb_k0(float v) {
-    // both branches evaluated and selected (no divergent branch per channel)
-    const float e_lin = v * (12.92f * 255.f);
-    const float e_pow = __builtin_fmaf(__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(v) * (1.f / 2.4f)),
-                                       1.055f * 255.f, -0.055f * 255.f);
-    const float e = v <= 0.0031308f ? e_lin : e_pow;
-    return tmin(tmax((int)(e + 0.25f), 0), 255);
-}
-
-__device__ __forceinline__ int srgb_code(const float4 *step, float v, float &dec) {
-    const int k0 = srgb_k0(v);
+// lut[i] is the code of i/4096; at most one threshold lies in each 1/4096 interval, so the
+// code of v is lut[i] or lut[i] + 1 for i = floor(4096 v).
+__device__ __forceinline__ int srgb_code(const float4 *step, const uint8_t *lut, float v, float &dec) {
+    const int i = tmin(tmax((int)(v * 4096.f), 0), 4095);
+    const int k0 = lut[i];
     const float4 st = step[k0];
     const bool up = v >= st.x;
     dec = up ? st.z : st.y;
@@ -1016,16 +1009,16 @@ __device__ __forceinline__ int srgb_code(const float4 *step, float v, float &dec
 }
 
 template <bool SRGB>
-__device__ __forceinline__ void blend_dec(const float4 *step, float *d, const float4 &src, float a) {
+__device__ __forceinline__ void blend_dec(const float4 *step, const uint8_t *lut, float *d, const float4 &src, float a) {
     const float oma = 1.f - a;
     const float v0 = src.x * a + d[0] * oma;
     const float v1 = src.y * a + d[1] * oma;
     const float v2 = src.z * a + d[2] * oma;
     const float va = a * a + d[3] * oma;
     if (SRGB) {
-        srgb_code(step, v0, d[0]);
-        srgb_code(step, v1, d[1]);
-        srgb_code(step, v2, d[2]);
+        srgb_code(step, lut, v0, d[0]);
+        srgb_code(step, lut, v1, d[1]);
+        srgb_code(step, lut, v2, d[2]);
     } else {
         d[0] = unorm_dec(unorm_code(v0));
         d[1] = unorm_dec(unorm_code(v1));
@@ -1035,11 +1028,11 @@ __device__ __forceinline__ void blend_dec(const float4 *step, float *d, const fl
 }
 
 template <bool SRGB>
-__device__ __forceinline__ uint32_t pack_dec(const float4 *step, const float *d) {
+__device__ __forceinline__ uint32_t pack_dec(const float4 *step, const uint8_t *lut, const float *d) {
     uint32_t o = 0u;
     float t;
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) o |= (uint32_t)(SRGB ? srgb_code(step, d[ch], t) : unorm_code(d[ch])) << (8 * ch);
+    for (int ch = 0; ch < 3; ++ch) o |= (uint32_t)(SRGB ? srgb_code(step, lut, d[ch], t) : unorm_code(d[ch])) << (8 * ch);
     return o | ((uint32_t)unorm_code(d[3]) << 24);
 }
 
@@ -1051,6 +1044,7 @@ struct CovArgs {
     const uint32_t *items, *icov;
     const float *dec_c;         // [256] colour decode, for the paint colours
     const float4 *step_c;       // [256] sRGB steps {thr[k+1], dec[k], dec[k+1], 0}
+    const uint8_t *lut_c;       // [4096] sRGB code of i/4096
     uint8_t *image;
     int pitch;
     // batched scenes: per scene {W, H, first tile, first cell row} and image byte offset
@@ -1085,6 +1079,7 @@ __device__ __noinline__ void scatter_big(const uint32_t *items, uint32_t ib, int
 template <int SPP, bool SRGB, int kCovGroup>
 __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
     __shared__ float4 s_step[SRGB ? 256 : 1];
+    __shared__ uint8_t s_lut[SRGB ? 4096 : 4];
     __shared__ float s_dec[256];
     __shared__ float4 l_src[4][kCovList];
     __shared__ uint2 l_ref[4][kCovList];
@@ -1093,6 +1088,7 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
         if (SRGB) s_step[i] = A.step_c[i];
         s_dec[i] = A.dec_c[i];
     }
+    for (int i = threadIdx.x; i < (SRGB ? 1024 : 0); i += 256) ((uint32_t *)s_lut)[i] = ((const uint32_t *)A.lut_c)[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // consecutive workgroups take consecutive tiles: the round-robin deal of workgroups to
@@ -1179,8 +1175,8 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
                         n0 = (pcv >> (6 * ((x - px0) * 2))) & 63u;
                         n1 = (pcv >> (6 * ((x - px0) * 2 + 1))) & 63u;
                     }
-                    blend_dec<SRGB>(s_step, d0, src, src.w * ((float)n0 / (float)SPP));
-                    blend_dec<SRGB>(s_step, d1, src, src.w * ((float)n1 / (float)SPP));
+                    blend_dec<SRGB>(s_step, s_lut, d0, src, src.w * ((float)n0 / (float)SPP));
+                    blend_dec<SRGB>(s_step, s_lut, d1, src, src.w * ((float)n1 / (float)SPP));
                 }
             }
             wave_sync_lds();
@@ -1222,8 +1218,8 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
 
     if (x >= W) return;
     const int y0 = 2 * r;
-    *(uint32_t *)(image + (size_t)y0 * pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, d0);
-    if (y0 + 1 < H) *(uint32_t *)(image + (size_t)(y0 + 1) * pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, d1);
+    *(uint32_t *)(image + (size_t)y0 * pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, s_lut, d0);
+    if (y0 + 1 < H) *(uint32_t *)(image + (size_t)(y0 + 1) * pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, s_lut, d1);
 }
 
 // Row-list records for the coverage compositor: one 16-byte load per listed bucket.
@@ -1243,13 +1239,13 @@ __global__ void k_row_records(int64_t NB, const uint32_t *__restrict__ n_ptr, co
 // decoded values of the codes.
 template <bool SRGB>
 __global__ void k_encode_values(int64_t n, const float *__restrict__ v, const float4 *__restrict__ step,
-                                uint8_t *__restrict__ code, float *__restrict__ dec) {
+                                const uint8_t *__restrict__ lut, uint8_t *__restrict__ code, float *__restrict__ dec) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float d;
     int k;
     if (SRGB) {
-        k = srgb_code(step, v[i], d);
+        k = srgb_code(step, lut, v[i], d);
     } else {
         k = unorm_code(v[i]);
         d = unorm_dec(k);
@@ -1261,8 +1257,8 @@ __global__ void k_encode_values(int64_t n, const float *__restrict__ v, const fl
 // Exhaustive comparison of the device requantisation with the threshold tables over every
 // float bit pattern in [lo, hi): counts disagreements of code or decoded value.
 template <bool SRGB>
-__global__ void k_encode_sweep(uint32_t lo, uint32_t hi, const float4 *__restrict__ step, const float *__restrict__ thr,
-                               const float *__restrict__ dec, unsigned long long *bad) {
+__global__ void k_encode_sweep(uint32_t lo, uint32_t hi, const float4 *__restrict__ step, const uint8_t *__restrict__ lut,
+                               const float *__restrict__ thr, const float *__restrict__ dec, unsigned long long *bad) {
     unsigned long long nbad = 0;
     for (uint64_t b = (uint64_t)lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < hi;
          b += (uint64_t)gridDim.x * blockDim.x) {
@@ -1274,7 +1270,7 @@ __global__ void k_encode_sweep(uint32_t lo, uint32_t hi, const float4 *__restric
         float d;
         int k;
         if (SRGB) {
-            k = srgb_code(step, v, d);
+            k = srgb_code(step, lut, v, d);
         } else {
             k = unorm_code(v);
             d = unorm_dec(k);
@@ -1423,7 +1419,7 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
         CovArgs C;
         C.W = f.W; C.H = f.H; C.tiles_x = tiles_x;
         C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec; C.rtile = f.rtile;
-        C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.step_c = t.step_c;
+        C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.step_c = t.step_c; C.lut_c = t.lut_c;
         C.image = f.image; C.pitch = f.pitch;
         C.nscenes = f.nscenes; C.ntiles = ntiles; C.stile = f.stile; C.simg = f.simg;
         dim3 g(blocks_for(ntiles, 4)), b(256);
@@ -1462,9 +1458,9 @@ void launch_encode_values(const Tables &t, int channel, const float *v, int64_t 
     if (n <= 0) return;
     dim3 g(blocks_for(n, 256)), b(256);
     if (channel == 0 && t.srgb)
-        hipLaunchKernelGGL(k_encode_values<true>, g, b, 0, s, n, v, t.step_c, code, dec);
+        hipLaunchKernelGGL(k_encode_values<true>, g, b, 0, s, n, v, t.step_c, t.lut_c, code, dec);
     else
-        hipLaunchKernelGGL(k_encode_values<false>, g, b, 0, s, n, v, t.step_c, code, dec);
+        hipLaunchKernelGGL(k_encode_values<false>, g, b, 0, s, n, v, t.step_c, t.lut_c, code, dec);
 }
 
 void launch_encode_sweep(const Tables &t, int channel, uint32_t lo, uint32_t hi, unsigned long long *bad,
@@ -1474,9 +1470,9 @@ void launch_encode_sweep(const Tables &t, int channel, uint32_t lo, uint32_t hi,
     const float *thr = channel == 0 ? t.thr_c : t.thr_a;
     const float *dec = channel == 0 ? t.dec_c : t.dec_a;
     if (srgb)
-        hipLaunchKernelGGL(k_encode_sweep<true>, dim3(8192), dim3(256), 0, s, lo, hi, t.step_c, thr, dec, bad);
+        hipLaunchKernelGGL(k_encode_sweep<true>, dim3(8192), dim3(256), 0, s, lo, hi, t.step_c, t.lut_c, thr, dec, bad);
     else
-        hipLaunchKernelGGL(k_encode_sweep<false>, dim3(8192), dim3(256), 0, s, lo, hi, t.step_c, thr, dec, bad);
+        hipLaunchKernelGGL(k_encode_sweep<false>, dim3(8192), dim3(256), 0, s, lo, hi, t.step_c, t.lut_c, thr, dec, bad);
 }
 
 void launch_debug_draw_list(const SceneDev &sc, const Tables &t, Frame &f, const uint32_t *moff, const uint32_t *ioff,
