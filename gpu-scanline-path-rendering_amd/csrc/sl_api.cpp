@@ -331,7 +331,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         return SL_OK;
     }
     if ((rc = ensure(h, h->b_ts, (size_t)F * 8)) || (rc = ensure(h, h->b_chains, (size_t)NCH * 18)) ||
-        (rc = ensure(h, h->b_chain_t, (size_t)F * 4)))
+        (rc = ensure(h, h->b_chain_t, (size_t)F * 4 + 64)))  // padded: read in aligned 16-byte groups
         return rc;
     f.ts = P_<int2>(h->b_ts);
     f.chains = P_<uint32_t>(h->b_chains);
@@ -374,9 +374,8 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     f.scratch_keys = P_<uint32_t>(h->b_scratch); f.items = P_<uint32_t>(h->b_items); f.imask = P_<uint32_t>(h->b_imask);
     f.icov = P_<uint32_t>(h->b_icov);
     // the rank array lives in the large-bucket scratch (used only later, by the resolve);
-    // the winding scan's input and output live in the item arrays the resolve overwrites
+    // the winding scan's output lives in the item masks the resolve overwrites
     f.frank = f.scratch_keys;
-    f.rec_swn = f.items;
     f.wscan = f.imask;
     f.bicnt = P_<uint32_t>(h->b_bicnt); f.bmcnt = P_<uint32_t>(h->b_bmcnt); f.bext = P_<uint32_t>(h->b_bext);
     f.big_list = P_<uint32_t>(h->b_big_list); f.nz_flag = P_<uint32_t>(h->b_nz_flag); f.nz_off = P_<uint32_t>(h->b_nz_off);

@@ -384,15 +384,14 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
     }
 }
 
-// Scatter pass: records at boff[bucket] + rank, and each record's scan winding (sign
-// extended) at the same slot for the winding-carry scan.
+// Scatter pass: records at boff[bucket] + rank.
 template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_t *__restrict__ tab,
                                                        const float2 *__restrict__ tv, const int2 *__restrict__ ts,
                                                        int64_t F, int W, int H, const int2 *__restrict__ prow,
                                                        const uint32_t *__restrict__ pbase,
                                                        const uint32_t *__restrict__ boff, const uint32_t *__restrict__ rank,
-                                                       uint32_t *rec_meta, uint32_t *rec_mask, uint32_t *rec_swn) {
+                                                       uint32_t *rec_meta, uint32_t *rec_mask) {
     constexpr int NW = Words<SPP>::n;
     constexpr int K = kFragPerThread;
     const int lane = threadIdx.x & 63;
@@ -429,7 +428,6 @@ __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_
         const Frag &g = f[j];
         const uint32_t cx = (uint32_t)((g.pos_x + 2) >> 1);
         rec_meta[slot] = cx | (((uint32_t)g.swn & 0xFFu) << 16) | ((g.wnc < 0 ? 1u : 0u) << 24);
-        rec_swn[slot] = (uint32_t)g.swn;
         if (NW == 4) {
             *(uint4 *)(rec_mask + (size_t)slot * 4) = make_uint4(g.m[0], g.m[1], g.m[2], g.m[3]);
         } else if (NW == 2) {
@@ -497,7 +495,7 @@ void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStrea
 #define SL_LAUNCH(N)                                                                                                \
     hipLaunchKernelGGL(k_frag_scatter<N>, g, b, 0, s, sc, t.pixel, (const float2 *)f.tvtx, (const int2 *)f.ts, f.F, \
                        f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, (const uint32_t *)f.boff,         \
-                       (const uint32_t *)f.frank, f.rec_meta, f.rec_mask, f.rec_swn)
+                       (const uint32_t *)f.frank, f.rec_meta, f.rec_mask)
     if (t.spp == 8) SL_LAUNCH(8);
     else if (t.spp == 16) SL_LAUNCH(16);
     else SL_LAUNCH(32);
@@ -506,7 +504,7 @@ void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStrea
 
 void launch_bucket_carry(Frame &f, hipStream_t s) {
     // records' scan windings -> exclusive scan -> carry at each bucket start
-    scan_exclusive_u32(f.rec_swn, f.wscan, (size_t)f.F, f.scan_tmp, s);
+    scan_exclusive_swn(f.rec_meta, f.wscan, (size_t)f.F, f.scan_tmp, s);
     hipLaunchKernelGGL(k_bucket_carry, dim3(blocks_for(f.NB + 1, 256)), dim3(256), 0, s, f.NB, (const uint32_t *)f.boff,
                        (const uint32_t *)f.wscan, f.bcarry);
 }

@@ -40,6 +40,15 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &total,
     return off + inc - v;
 }
 
+// Scan input: the value itself, or (SWN) the sign-extended scan winding held in bits
+// 16..23 of a fragment record's meta word.
+template <bool SWN>
+__device__ __forceinline__ uint32_t scan_in(const uint32_t *in, size_t i) {
+    const uint32_t v = in[i];
+    return SWN ? (uint32_t)(int)(int8_t)(v >> 16) : v;
+}
+
+template <bool SWN>
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t *in, size_t n, uint32_t *tile_sum) {
     __shared__ uint32_t lds[4];
     size_t base = (size_t)blockIdx.x * kScanTile;
@@ -47,7 +56,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t *in
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         size_t i = base + (size_t)k * kScanThreads + threadIdx.x;
-        if (i < n) s += in[i];
+        if (i < n) s += scan_in<SWN>(in, i);
     }
     uint32_t tot;
     block_excl_scan(s, tot, lds);
@@ -68,6 +77,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_top(uint32_t *tile_sum, s
     if (threadIdx.x == 0) *out_total = carry;
 }
 
+template <bool SWN>
 __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t *in, uint32_t *out, size_t n,
                                                              const uint32_t *tile_off) {
     __shared__ uint32_t lds[4];
@@ -77,7 +87,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t *in, 
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         size_t i = base + k;
-        v[k] = i < n ? in[i] : 0u;
+        v[k] = i < n ? scan_in<SWN>(in, i) : 0u;
         s += v[k];
     }
     uint32_t tot;
@@ -520,13 +530,32 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
     const float a0 = side ? cu.y0 : cu.x0, a1 = side ? cu.y1 : cu.x1;
     const float a2 = side ? cu.y2 : cu.x2, a3 = side ? cu.y3 : cu.x3;
     float *out = chain_t + off + (side ? nx : 0u);
+    // scalar stores up to a 16-byte boundary, then whole float4 stores, then the tail
+    const uint32_t head = (uint32_t)((4u - (((uintptr_t)out >> 2) & 3u)) & 3u);
     float t_min = t0m;
+    float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t j = 0; j < cnt; ++j) {
         const float t = solve_cut(cu.type, cu.w, a0, a1, a2, a3, cc, t_min, t1m);
         const float tm = bf((fb(t) & 0xFFFFFFFCu) | (uint32_t)side);
-        out[j] = tm;
+        if (j < head) {
+            out[j] = tm;
+        } else {
+            const uint32_t ph = (j - head) & 3u;
+            if (ph == 0) q4.x = tm;
+            else if (ph == 1) q4.y = tm;
+            else if (ph == 2) q4.z = tm;
+            else q4.w = tm;
+            if (ph == 3) *(float4 *)(out + j - 3) = q4;
+        }
         t_min = tm;
         cc += dd;
+    }
+    if (cnt > head) {
+        const uint32_t rem = (cnt - head) & 3u;  // values still held in q4
+        const uint32_t j0 = cnt - rem;
+        if (rem > 0) out[j0] = q4.x;
+        if (rem > 1) out[j0 + 1] = q4.y;
+        if (rem > 2) out[j0 + 2] = q4.z;
     }
 }
 
@@ -562,28 +591,51 @@ __device__ __forceinline__ PieceRef piece_ref(const SceneDev &sc, const float2 *
     return r;
 }
 
+// Sequential reader of a chain: one 16-byte load per aligned group of four values (the
+// chain buffer is 16-byte aligned and padded, so whole groups stay inside it).
+struct ChainReader {
+    const float *p;
+    const float4 *blk;
+    float4 buf;
+    __device__ __forceinline__ void init(const float *q) {
+        p = q;
+        blk = nullptr;
+    }
+    __device__ __forceinline__ float at(uint32_t i) {
+        const uintptr_t a = (uintptr_t)(p + i);
+        const float4 *b = (const float4 *)(a & ~(uintptr_t)15);
+        if (b != blk) {
+            blk = b;
+            buf = *b;
+        }
+        const uint32_t c = (uint32_t)(a >> 2) & 3u;
+        return c == 0 ? buf.x : (c == 1 ? buf.y : (c == 2 ? buf.z : buf.w));
+    }
+};
+
 __device__ __forceinline__ void walk_piece(const PieceRef &r, const float *__restrict__ chain_t, int2 *ts) {
     int2 *out = ts + r.base;
-    const float *X = chain_t + r.base;
     const uint32_t nx = r.nx, ny = r.ny;
-    const float *Y = X + nx;
+    ChainReader X, Y;
+    X.init(chain_t + r.base);
+    Y.init(chain_t + r.base + nx);
     const float t0m = r.t0m;
     const float two = 2.f, two1 = bf(fb(2.f) | 1u);
     float xs, ys;
     uint32_t xi = 0, yi = 0, rx, ry;
     if (nx == 0) { xs = two; ys = t0m; rx = 0; ry = ny; }
     else if (ny == 0) { xs = t0m; ys = two1; rx = nx; ry = 0; }
-    else { xs = X[0]; xi = 1; rx = nx - 1; ys = t0m; ry = ny; }
+    else { xs = X.at(0); xi = 1; rx = nx - 1; ys = t0m; ry = ny; }
     uint32_t last = fb(-1.f);
     const uint32_t n_loop = nx + ny + 1;
     for (uint32_t it = 0; it < n_loop; ++it) {
         float v;
         if (xs <= ys) {
             v = xs;
-            if (rx > 0) { xs = X[xi++]; rx--; } else { xs = two; }
+            if (rx > 0) { xs = X.at(xi++); rx--; } else { xs = two; }
         } else {
             v = ys;
-            if (ry > 0) { ys = Y[yi++]; ry--; } else { ys = two1; }
+            if (ry > 0) { ys = Y.at(yi++); ry--; } else { ys = two1; }
         }
         uint32_t o = fb(v);
         if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
@@ -678,15 +730,25 @@ __global__ void __launch_bounds__(64) k_merge_long(SceneDev sc, const float2 *__
 
 size_t scan_tmp_elems_needed(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
-void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
+template <bool SWN>
+static void scan_impl(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
     size_t ntiles = (n + kScanTile - 1) / kScanTile;
     if (ntiles == 0) {
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tmp, (size_t)0, out + n);
         return;
     }
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_reduce<SWN>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, n, tmp);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tmp, ntiles, out + n);
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, out, n, (const uint32_t *)tmp);
+    hipLaunchKernelGGL(k_scan_down<SWN>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, out, n,
+                       (const uint32_t *)tmp);
+}
+
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
+    scan_impl<false>(in, out, n, tmp, s);
+}
+
+void scan_exclusive_swn(const uint32_t *meta, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
+    scan_impl<true>(meta, out, n, tmp, s);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {

@@ -79,7 +79,6 @@ struct Frame {
     uint32_t *bcarry;        // [NB+1]
     uint32_t *bcursor;       // [NB]
     uint32_t *frank;         // [F] rank of each fragment in its bucket (~0: none)
-    uint32_t *rec_swn;       // [F] scan winding of each record, sign-extended
     uint32_t *wscan;         // [F+1] exclusive scan of rec_swn
     uint32_t *rec_meta;      // [F] (valid fragments only are written)
     uint32_t *rec_mask;      // [F * words]
@@ -128,6 +127,8 @@ struct Frame {
 // ---- launchers (sl_kernels.hip) ----
 size_t scan_tmp_elems_needed(size_t n);
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s);
+// exclusive scan of the sign-extended scan windings in fragment meta words (bits 16..23)
+void scan_exclusive_swn(const uint32_t *meta, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s);
 
 void launch_path_init(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_transform(const SceneDev &sc, Frame &f, hipStream_t s);
