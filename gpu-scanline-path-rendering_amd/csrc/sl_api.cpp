@@ -256,6 +256,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) f.m[4 * i + j] = h->m[4 * j + i];
     f.srgb = h->srgb; f.ms = h->ms;
+    f.keep_masks = (h->ms || h->debug) ? 1 : 0;
     f.pitch = h->W * 4;
     f.R = h->nscenes ? h->batch_rows : (int64_t)(h->H + 1) / 2;
     f.nscenes = h->nscenes;

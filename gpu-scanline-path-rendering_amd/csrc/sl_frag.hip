@@ -384,7 +384,9 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
     }
 }
 
-// Scatter pass: records at boff[bucket] + rank.
+// Scatter pass: records at boff[bucket] + rank.  A path's fragments are neighbours in the
+// input and land in the same few buckets, so the XCD-contiguous block order keeps their
+// partial-line writes in one L2.
 template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_t *__restrict__ tab,
                                                        const float2 *__restrict__ tv, const int2 *__restrict__ ts,
@@ -395,7 +397,7 @@ __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_
     constexpr int NW = Words<SPP>::n;
     constexpr int K = kFragPerThread;
     const int lane = threadIdx.x & 63;
-    const int64_t i0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64 * K + lane;
+    const int64_t i0 = ((int64_t)xcd_chunk(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64 * K + lane;
     FragIn in[K];
     bool live[K];
     int2 pr[K];

@@ -9,6 +9,15 @@ namespace sl {
 
 // Per-frame device views handed to the kernel launchers.  Every pointer is a device
 // pointer owned by the handle's arena.
+// Block order for the 8 XCDs: hardware hands consecutive workgroups to the XCDs in turn,
+// so workgroup b runs on XCD b % 8; mapping it to chunk start(b % 8) + b / 8 gives each
+// XCD one contiguous range of a uniform-cost grid, so neighbouring output records are
+// written through one XCD's L2 (the mapping is a bijection whatever the deal really is).
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t n) {
+    const uint32_t x = b & 7u, i = b >> 3, q = n >> 3, r = n & 7u;
+    return x * q + (x < r ? x : r) + i;
+}
+
 struct SceneDev {
     uint32_t P, C, V;
     const float2 *vtx;       // [V] vertices in unified (orientation-fixed) order
@@ -119,6 +128,7 @@ struct Frame {
     uint8_t *image;          // [H * pitch]
     int pitch;
     int srgb, ms;
+    int keep_masks;          // resolve stores the merged masks (multisample output, debug dumps)
     // scan scratch
     uint32_t *scan_tmp;
     size_t scan_tmp_elems;

@@ -144,7 +144,7 @@ struct ResolveArgs {
     const int4 *pcanvas;     // batched scenes: per-path canvas (W, H, row base, scene)
     const uint32_t *scarry;  // batched scenes: winding-scan carry at each scene's first bucket
     uint32_t *items;
-    uint32_t *imask;
+    uint32_t *imask;         // null when no consumer needs the merged masks
     uint32_t *icov;
     uint32_t *bicnt;
     uint32_t *bmcnt;
@@ -286,7 +286,7 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
         }
         if (merged) {
             A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
-            store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
+            if (A.imask) store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
             A.icov[ib + idx + (span ? 1 : 0)] = cell_counts<SPP>(res);
         }
         const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
@@ -400,7 +400,7 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
     }
     if (merged) {
         A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
-        store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
+        if (A.imask) store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
         A.icov[ib + idx + (span ? 1 : 0)] = cell_counts<SPP>(res);
     }
     const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
@@ -566,7 +566,7 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
             }
             if (merged) {
                 A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
-                store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
+                if (A.imask) store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
                 A.icov[ib + idx + (span ? 1 : 0)] = cell_counts<SPP>(res);
             }
             const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
@@ -1343,7 +1343,10 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
     A.NB = f.NB; A.binfo = f.binfo; A.boff = f.boff; A.bcarry = f.bcarry; A.frule = sc.frule;
     A.rec_meta = f.rec_meta; A.rec_mask = f.rec_mask; A.scratch = f.scratch_keys; A.W = f.W;
     A.pcanvas = sc.pcanvas; A.scarry = f.scarry;
-    A.items = f.items; A.imask = f.imask; A.icov = f.icov; A.bicnt = f.bicnt; A.bmcnt = f.bmcnt; A.bext = f.bext;
+    // the merged masks feed the multisample compositor and the debug draw list; the
+    // coverage compositor reads only the per-pixel sample counts
+    A.items = f.items; A.imask = f.keep_masks ? f.imask : nullptr; A.icov = f.icov; A.bicnt = f.bicnt;
+    A.bmcnt = f.bmcnt; A.bext = f.bext;
     A.btile = f.btile;
     A.big_list = f.big_list; A.big_count = f.big_count; A.err = f.err_flag;
     launch_fill_u32(f.big_count, 0u, 1, s);
