@@ -335,9 +335,29 @@ __device__ __forceinline__ void load_fragments(const SceneDev &sc, const float2 
 
 // Count pass: every fragment that lands in a bucket (a valid row, or an out-of-canvas row
 // whose scan winding still feeds the winding carry) takes its rank in the bucket from the
-// bucket counter: one returning atomic per distinct bucket of the wave, the lanes of a
-// bucket numbered by lane order.  The rank lets the scatter pass place records without
-// atomics.
+// bucket counter, which lets the scatter pass place records without atomics.  A block's
+// fragments are consecutive, so they belong to a few neighbouring paths and their buckets
+// form a short window of bucket ids: the block counts them in LDS (the LDS atomic returns
+// the rank inside the block), then one thread per non-empty bucket of the window reserves
+// the block's share with one returning device atomic.  Ranks inside a bucket are in no
+// particular order; the resolve pass orders a bucket's records by cell and its results do
+// not depend on the order of records within a cell.  Windows wider than the LDS array
+// fall back to one atomic per distinct bucket of a wave.
+constexpr uint32_t kCountWindow = 2048;
+
+__device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = umin(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = umax(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
 template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t *__restrict__ tab,
                                                      const float2 *__restrict__ tv, const int2 *__restrict__ ts, int64_t F,
@@ -345,32 +365,75 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
                                                      const uint32_t *__restrict__ pbase, uint32_t *bcount, uint32_t *rank,
                                                      uint32_t *err) {
     constexpr int K = kFragPerThread;
-    const int lane = threadIdx.x & 63;
+    __shared__ uint32_t s_cnt[kCountWindow];
+    __shared__ uint32_t s_lo[4], s_hi[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (int)(blockDim.x >> 6);
     const int64_t i0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64 * K + lane;
     FragIn in[K];
     bool live[K];
     int2 pr[K];
     uint32_t pb[K];
     load_fragments<K>(sc, tv, ts, F, i0, prow, pbase, in, live, pr, pb);
+    bool act[K];
+    uint32_t b[K];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        bool act = false;
-        uint32_t b = 0;
+        act[j] = false;
+        b[j] = 0;
         if (live[j]) {
             FragMaskIn mi;
             const Frag f = frag_geometry<SPP, false>(sc, in[j], W, H, mi);
             bool ok;
-            b = (uint32_t)bucket_of(pr[j], pb[j], f.row, ok);
+            b[j] = (uint32_t)bucket_of(pr[j], pb[j], f.row, ok);
             if (!ok) atomicOr(err, 1u);
-            act = ok && (f.row >= 0 || f.swn != 0);
+            act[j] = ok && (f.row >= 0 || f.swn != 0);
+            if (act[j]) {
+                lo = umin(lo, b[j]);
+                hi = umax(hi, b[j]);
+            }
         }
-        uint64_t pend = __ballot(act);
+    }
+    lo = wave_min_u32(lo);
+    hi = wave_max_u32(hi);
+    if (lane == 0) {
+        s_lo[wv] = lo;
+        s_hi[wv] = hi;
+    }
+    __syncthreads();
+    lo = s_lo[0];
+    hi = s_hi[0];
+    for (int k = 1; k < nwv; ++k) {
+        lo = umin(lo, s_lo[k]);
+        hi = umax(hi, s_hi[k]);
+    }
+    if (lo <= hi && hi - lo < kCountWindow) {
+        const uint32_t span = hi - lo + 1;
+        for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) s_cnt[i] = 0u;
+        __syncthreads();
+        uint32_t lr[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) lr[j] = act[j] ? atomicAdd(&s_cnt[b[j] - lo], 1u) : 0u;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
+            const uint32_t c = s_cnt[i];
+            if (c) s_cnt[i] = atomicAdd(&bcount[lo + i], c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (live[j]) rank[i0 + 64 * j] = act[j] ? s_cnt[b[j] - lo] + lr[j] : 0xFFFFFFFFu;
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        uint64_t pend = __ballot(act[j]);
         uint32_t base = 0, r = 0;
         int my_leader = 0;
         while (pend) {
             const int leader = __ffsll((long long)pend) - 1;
-            const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, leader);
-            const bool mine = ((pend >> lane) & 1ull) && b == lb;
+            const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b[j], leader);
+            const bool mine = ((pend >> lane) & 1ull) && b[j] == lb;
             const uint64_t grp = __ballot(mine);
             if (mine) {
                 my_leader = leader;
@@ -380,7 +443,7 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
             pend &= ~grp;
         }
         base = (uint32_t)__shfl((int)base, my_leader, 64);
-        if (live[j]) rank[i0 + 64 * j] = act ? base + r : 0xFFFFFFFFu;
+        if (live[j]) rank[i0 + 64 * j] = act[j] ? base + r : 0xFFFFFFFFu;
     }
 }
 

@@ -397,7 +397,9 @@ __device__ __forceinline__ uint32_t wave_excl_scan_dpp(uint32_t v, uint32_t &tot
     return (uint32_t)x - v;
 }
 
-__global__ void __launch_bounds__(256) k_chain_list(uint32_t C, const uint32_t *__restrict__ nchain,
+constexpr int kChainListThreads = 1024;
+
+__global__ void __launch_bounds__(kChainListThreads) k_chain_list(uint32_t C, const uint32_t *__restrict__ nchain,
                                                     const uint32_t *__restrict__ pcnt, uint32_t cstride,
                                                     uint32_t pstride, uint32_t *clist, uint32_t *plist,
                                                     uint32_t *counts /* [3] chains, [3] pieces */) {
@@ -418,15 +420,32 @@ __global__ void __launch_bounds__(256) k_chain_list(uint32_t C, const uint32_t *
         if (ny) n[len_class(ny)]++;
         n[3 + len_class(nx + ny + 1)]++;
     }
+    // wave scans, then one reservation per class and block (the six counters are shared by
+    // the whole grid, so fewer, larger reservations keep them from serialising)
+    __shared__ uint32_t s_off[kChainListThreads / 64][6];
+    const int wv = threadIdx.x >> 6;
     uint32_t off[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         uint32_t tot;
-        const uint32_t ex = wave_excl_scan_dpp(n[q], tot);
-        uint32_t base = 0;
-        if (lane == 0 && tot) base = atomicAdd(&counts[q], tot);
-        off[q] = (uint32_t)__builtin_amdgcn_readlane((int)base, 0) + ex;
+        off[q] = wave_excl_scan_dpp(n[q], tot);
+        if (lane == 0) s_off[wv][q] = tot;
     }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int q = (int)threadIdx.x;
+        uint32_t acc = 0;
+        for (int k = 0; k < kChainListThreads / 64; ++k) {
+            const uint32_t t = s_off[k][q];
+            s_off[k][q] = acc;
+            acc += t;
+        }
+        const uint32_t base = acc ? atomicAdd(&counts[q], acc) : 0u;
+        for (int k = 0; k < kChainListThreads / 64; ++k) s_off[k][q] += base;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 6; ++q) off[q] += s_off[wv][q];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (k >= np) break;
@@ -783,7 +802,8 @@ void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.C) return;
     const uint32_t cstride = (uint32_t)f.NCH, pstride = (uint32_t)(f.NCH / 2);
     launch_fill_u32(f.lcount, 0u, 6, s);
-    hipLaunchKernelGGL(k_chain_list, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc.C, (const uint32_t *)f.nchain,
+    hipLaunchKernelGGL(k_chain_list, dim3(blocks_for(sc.C, kChainListThreads)), dim3(kChainListThreads), 0, s, sc.C,
+                       (const uint32_t *)f.nchain,
                        (const uint32_t *)f.pcnt, cstride, pstride, f.chains, f.plist, f.lcount);
     if (f.NCH)
         hipLaunchKernelGGL(k_chains, dim3(blocks_for(f.NCH, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
