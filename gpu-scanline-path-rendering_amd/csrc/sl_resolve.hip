@@ -103,6 +103,66 @@ struct Stencil {
     }
 };
 
+// The same result for small groups, bit-sliced.  The byte lanes of the stencil word start
+// at w * 0x01010101 and gain +/- 0x01 at set bits, so the word is sum_j c_j * 256^j
+// (mod 2^32) with c_j the sample's true count w + d_j.  While every c_j lies in
+// [-128, 127], adding 0x80808080 leaves byte j = c_j + 128 with no carry between lanes,
+// and the tests read c_j mod 128 != 0 (nonzero) or c_j & 1 (even-odd).  For g <= 7
+// members and |w| <= 120, c_j lies in [-127, 127], so it is enough to count d_j in
+// [-7, 7] in four two's-complement bit planes: nonzero is d_j != -w, even-odd is
+// d_j ^ w in plane 0.  Groups outside that range take the byte-lane stencil.
+constexpr int kSlicedMaxGroup = 7;
+constexpr int kSlicedMaxWinding = 120;
+
+__device__ __forceinline__ bool sliced_ok(int g, int8_t w) {
+    return g <= kSlicedMaxGroup && w >= -kSlicedMaxWinding && w <= kSlicedMaxWinding;
+}
+
+template <int NW>
+struct Sliced {
+    uint32_t d[4][NW];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int k = 0; k < NW; ++k) d[p][k] = 0u;
+    }
+    // d += 1 (or -= 1) at the set bits: ripple carry (borrow) through the planes
+    __device__ __forceinline__ void add(const uint32_t *m, bool neg) {
+        const uint32_t nm = neg ? ~0u : 0u;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            uint32_t c = m[k];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const uint32_t t = (d[p][k] ^ nm) & c;
+                d[p][k] ^= c;
+                c = t;
+            }
+        }
+    }
+    __device__ __forceinline__ void finish(int rule, int8_t w, uint32_t *out) const {
+        if (rule != 0) {
+            const uint32_t odd = (w & 1) ? ~0u : 0u;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) out[k] = d[0][k] ^ odd;
+            return;
+        }
+        const int t = -(int)w;
+        const bool hit = t >= -8 && t <= 7;  // d_j in [-7, 7] can equal -w only then
+        uint32_t tp[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) tp[p] = ((t >> p) & 1) ? ~0u : 0u;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            uint32_t ne = 0u;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) ne |= d[p][k] ^ tp[p];
+            out[k] = hit ? ne : ~0u;
+        }
+    }
+};
+
 // Sample counts of the four pixels of a resolved cell mask, 6 bits each at bit 6*(dx*2+dy).
 template <int SPP>
 __device__ __forceinline__ uint32_t cell_counts(const uint32_t *m) {
@@ -187,6 +247,29 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
 __device__ __forceinline__ int scan_add(int x) { return dpp_scan(x, 0, OpAdd()); }
 __device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, 63); }
 
+// Merged mask of the cell group at sorted positions [p, e) (records staged in LDS).
+template <int NW>
+__device__ __forceinline__ void group_mask(const uint32_t *order, const uint32_t *meta, const uint32_t *msk, int p,
+                                           int e, int8_t w_at, int rule, uint32_t *res) {
+    if (sliced_ok(e - p, w_at)) {
+        Sliced<NW> st;
+        st.init();
+        for (int q = p; q < e; ++q) {
+            const uint32_t iq = order[q];
+            st.add(msk + iq * NW, ((meta[iq] >> 24) & 1u) != 0u);
+        }
+        st.finish(rule, w_at, res);
+    } else {
+        Stencil<NW> st;
+        st.init(w_at);
+        for (int q = p; q < e; ++q) {
+            const uint32_t iq = order[q];
+            st.add(msk + iq * NW, ((meta[iq] >> 24) & 1u) != 0u);
+        }
+        st.finish(rule, res);
+    }
+}
+
 // Resolve one (path, row) bucket of n <= 64*S fragments with one wavefront: stage the
 // records in LDS, rank them by (cell x, index), then walk the sorted order in chunks of
 // 64 carrying the winding number and item count.  Per cell group, the first lane sums
@@ -264,15 +347,9 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
         const bool merged = head && x >= 0 && x < W;
         uint32_t res[NW];
         if (merged) {
-            Stencil<NW> st;
-            st.init(w_at);
-            for (int q = p; q < n; ++q) {
-                const uint32_t iq = order[q];
-                const uint32_t mq = meta[iq];
-                if ((int)(mq & 0xFFFFu) != cx) break;
-                st.add(msk + iq * NW, ((mq >> 24) & 1u) != 0u);
-            }
-            st.finish(rule, res);
+            int e = p + 1;
+            while (e < n && (int)(meta[order[e]] & 0xFFFFu) == cx) ++e;
+            group_mask<NW>(order, meta, msk, p, e, w_at, rule, res);
         }
         const int prev_x = 2 * prev_cx - 2;
         const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
@@ -377,17 +454,13 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
     const int8_t w_at = (int8_t)(int)(wcarry + (uint32_t)(incl - swn));
     const int x = 2 * cx - 2;
     const bool merged = head && x >= 0 && x < W;
+    // the group of a head lane ends at the next head (or the bucket end)
+    const uint64_t heads = __ballot(head);
     uint32_t res[NW];
     if (merged) {
-        Stencil<NW> st;
-        st.init(w_at);
-        for (int q = p; q < n; ++q) {
-            const uint32_t iq = order[q];
-            const uint32_t mq = meta[iq];
-            if ((int)(mq & 0xFFFFu) != cx) break;
-            st.add(msk + iq * NW, ((mq >> 24) & 1u) != 0u);
-        }
-        st.finish(rule, res);
+        const uint64_t later = p < 63 ? (heads & (~0ull << (p + 1))) : 0ull;
+        const int e = later ? (int)__builtin_ctzll(later) : n;
+        group_mask<NW>(order, meta, msk, p, e, w_at, rule, res);
     }
     const int prev_x = 2 * prev_cx - 2;
     const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
