@@ -19,21 +19,16 @@ constexpr int CT_LINEAR = 2, CT_QUAD = 3, CT_CUBIC = 4, CT_RATIONAL = 0x13;
 __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float bf(uint32_t u) { return __uint_as_float(u); }
 
+// Saturating conversion of an integral float: NaN -> 0, beyond the int range -> INT_MIN /
+// INT_MAX.  Written as a clamp into range plus selects, so it compiles without branches.
+__device__ __forceinline__ int32_t sat_i32(float f) {
+    int32_t r = (int32_t)fminf(fmaxf(f, -2147483648.0f), 2147483520.0f);  // largest float < 2^31
+    r = (f >= 2147483648.0f) ? INT_MAX : r;
+    return (f != f) ? 0 : r;
+}
 // __float2int_rd semantics (ras_cut.h:639-657)
-__device__ __forceinline__ int32_t floor_i32(float x) {
-    float f = floorf(x);
-    if (f != f) return 0;
-    if (f >= 2147483648.0f) return INT_MAX;
-    if (f < -2147483648.0f) return INT_MIN;
-    return (int32_t)f;
-}
-__device__ __forceinline__ int32_t trunc_i32(float x) {
-    float f = truncf(x);
-    if (f != f) return 0;
-    if (f >= 2147483648.0f) return INT_MAX;
-    if (f < -2147483648.0f) return INT_MIN;
-    return (int32_t)f;
-}
+__device__ __forceinline__ int32_t floor_i32(float x) { return sat_i32(floorf(x)); }
+__device__ __forceinline__ int32_t trunc_i32(float x) { return sat_i32(truncf(x)); }
 __device__ __forceinline__ int32_t trunc_i32(double x) {
     double f = trunc(x);
     if (f != f) return 0;
