@@ -377,7 +377,6 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     // the rank array lives in the large-bucket scratch (used only later, by the resolve);
     // the winding scan's output lives in the item masks the resolve overwrites
     f.frank = f.scratch_keys;
-    f.wscan = f.imask;
     f.bicnt = P_<uint32_t>(h->b_bicnt); f.bmcnt = P_<uint32_t>(h->b_bmcnt); f.bext = P_<uint32_t>(h->b_bext);
     f.big_list = P_<uint32_t>(h->b_big_list); f.nz_flag = P_<uint32_t>(h->b_nz_flag); f.nz_off = P_<uint32_t>(h->b_nz_off);
     f.nz_keys[0] = P_<uint32_t>(h->b_nzk0); f.nz_keys[1] = P_<uint32_t>(h->b_nzk1);

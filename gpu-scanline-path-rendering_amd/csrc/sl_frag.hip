@@ -342,7 +342,8 @@ __device__ __forceinline__ void load_fragments(const SceneDev &sc, const float2 
 // the block's share with one returning device atomic.  Ranks inside a bucket are in no
 // particular order; the resolve pass orders a bucket's records by cell and its results do
 // not depend on the order of records within a cell.  Windows wider than the LDS array
-// fall back to one atomic per distinct bucket of a wave.
+// fall back to one atomic per distinct bucket of a wave.  The same window sums the
+// buckets' scan windings (mod 2^32) for the winding carry.
 constexpr uint32_t kCountWindow = 2048;
 
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
@@ -362,10 +363,10 @@ template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t *__restrict__ tab,
                                                      const float2 *__restrict__ tv, const int2 *__restrict__ ts, int64_t F,
                                                      int W, int H, const int2 *__restrict__ prow,
-                                                     const uint32_t *__restrict__ pbase, uint32_t *bcount, uint32_t *rank,
-                                                     uint32_t *err) {
+                                                     const uint32_t *__restrict__ pbase, uint32_t *bcount, uint32_t *bwsum,
+                                                     uint32_t *rank, uint32_t *err) {
     constexpr int K = kFragPerThread;
-    __shared__ uint32_t s_cnt[kCountWindow];
+    __shared__ uint32_t s_cnt[kCountWindow], s_sw[kCountWindow];
     __shared__ uint32_t s_lo[4], s_hi[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (int)(blockDim.x >> 6);
     const int64_t i0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64 * K + lane;
@@ -376,11 +377,13 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
     load_fragments<K>(sc, tv, ts, F, i0, prow, pbase, in, live, pr, pb);
     bool act[K];
     uint32_t b[K];
+    int sw[K];
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         act[j] = false;
         b[j] = 0;
+        sw[j] = 0;
         if (live[j]) {
             FragMaskIn mi;
             const Frag f = frag_geometry<SPP, false>(sc, in[j], W, H, mi);
@@ -388,6 +391,7 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
             b[j] = (uint32_t)bucket_of(pr[j], pb[j], f.row, ok);
             if (!ok) atomicOr(err, 1u);
             act[j] = ok && (f.row >= 0 || f.swn != 0);
+            sw[j] = act[j] ? (int)f.swn : 0;
             if (act[j]) {
                 lo = umin(lo, b[j]);
                 hi = umax(hi, b[j]);
@@ -409,15 +413,22 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
     }
     if (lo <= hi && hi - lo < kCountWindow) {
         const uint32_t span = hi - lo + 1;
-        for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) s_cnt[i] = 0u;
+        for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
+            s_cnt[i] = 0u;
+            s_sw[i] = 0u;
+        }
         __syncthreads();
         uint32_t lr[K];
 #pragma unroll
-        for (int j = 0; j < K; ++j) lr[j] = act[j] ? atomicAdd(&s_cnt[b[j] - lo], 1u) : 0u;
+        for (int j = 0; j < K; ++j) {
+            lr[j] = act[j] ? atomicAdd(&s_cnt[b[j] - lo], 1u) : 0u;
+            if (sw[j]) atomicAdd(&s_sw[b[j] - lo], (uint32_t)sw[j]);
+        }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
-            const uint32_t c = s_cnt[i];
+            const uint32_t c = s_cnt[i], w = s_sw[i];
             if (c) s_cnt[i] = atomicAdd(&bcount[lo + i], c);
+            if (w) atomicAdd(&bwsum[lo + i], w);
         }
         __syncthreads();
 #pragma unroll
@@ -427,6 +438,7 @@ __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t 
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
+        if (sw[j]) atomicAdd(&bwsum[b[j]], (uint32_t)sw[j]);
         uint64_t pend = __ballot(act[j]);
         uint32_t base = 0, r = 0;
         int my_leader = 0;
@@ -503,14 +515,6 @@ __global__ void __launch_bounds__(256) k_frag_scatter(SceneDev sc, const uint32_
     }
 }
 
-// Winding carry of every bucket: the exclusive sum (mod 2^32) of the scan windings of all
-// records before it, bucket order being the reference's fragment order.
-__global__ void k_bucket_carry(int64_t NB, const uint32_t *__restrict__ boff, const uint32_t *__restrict__ wscan,
-                               uint32_t *bcarry) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b <= NB) bcarry[b] = wscan[boff[b]];
-}
-
 template <int SPP>
 __global__ void __launch_bounds__(256) k_debug_fragments(SceneDev sc, const uint32_t *__restrict__ tab,
                                                           const float2 *__restrict__ tv, const int2 *__restrict__ ts,
@@ -546,7 +550,8 @@ void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_
     dim3 g(blocks_for(f.F, fb * kFragPerThread)), b(fb);
 #define SL_LAUNCH(N)                                                                                              \
     hipLaunchKernelGGL(k_frag_count<N>, g, b, 0, s, sc, t.pixel, (const float2 *)f.tvtx, (const int2 *)f.ts, f.F, \
-                       f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, f.bcount, f.frank, f.err_flag)
+                       f.W, f.H, (const int2 *)f.prow, (const uint32_t *)f.pbase, f.bcount, f.bwsum, f.frank, \
+                       f.err_flag)
     if (t.spp == 8) SL_LAUNCH(8);
     else if (t.spp == 16) SL_LAUNCH(16);
     else SL_LAUNCH(32);
@@ -568,10 +573,9 @@ void launch_frag_scatter(const SceneDev &sc, const Tables &t, Frame &f, hipStrea
 }
 
 void launch_bucket_carry(Frame &f, hipStream_t s) {
-    // records' scan windings -> exclusive scan -> carry at each bucket start
-    scan_exclusive_swn(f.rec_meta, f.wscan, (size_t)f.F, f.scan_tmp, s);
-    hipLaunchKernelGGL(k_bucket_carry, dim3(blocks_for(f.NB + 1, 256)), dim3(256), 0, s, f.NB, (const uint32_t *)f.boff,
-                       (const uint32_t *)f.wscan, f.bcarry);
+    // winding carry of every bucket: the exclusive sum (mod 2^32) of the scan windings of
+    // all records in the buckets before it, bucket order being the reference's fragment order
+    scan_exclusive_u32(f.bwsum, f.bcarry, (size_t)f.NB, f.scan_tmp, s);
 }
 
 void launch_debug_fragments(const SceneDev &sc, const Tables &t, Frame &f, int32_t *yx, int32_t *pid, int8_t *wn,

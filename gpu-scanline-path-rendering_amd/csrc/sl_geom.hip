@@ -40,15 +40,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &total,
     return off + inc - v;
 }
 
-// Scan input: the value itself, or (SWN) the sign-extended scan winding held in bits
-// 16..23 of a fragment record's meta word.
-template <bool SWN>
-__device__ __forceinline__ uint32_t scan_in(const uint32_t *in, size_t i) {
-    const uint32_t v = in[i];
-    return SWN ? (uint32_t)(int)(int8_t)(v >> 16) : v;
-}
-
-template <bool SWN>
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t *in, size_t n, uint32_t *tile_sum) {
     __shared__ uint32_t lds[4];
     size_t base = (size_t)blockIdx.x * kScanTile;
@@ -56,7 +47,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t *in
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         size_t i = base + (size_t)k * kScanThreads + threadIdx.x;
-        if (i < n) s += scan_in<SWN>(in, i);
+        if (i < n) s += in[i];
     }
     uint32_t tot;
     block_excl_scan(s, tot, lds);
@@ -77,7 +68,6 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_top(uint32_t *tile_sum, s
     if (threadIdx.x == 0) *out_total = carry;
 }
 
-template <bool SWN>
 __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t *in, uint32_t *out, size_t n,
                                                              const uint32_t *tile_off) {
     __shared__ uint32_t lds[4];
@@ -87,7 +77,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t *in, 
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         size_t i = base + k;
-        v[k] = i < n ? scan_in<SWN>(in, i) : 0u;
+        v[k] = i < n ? in[i] : 0u;
         s += v[k];
     }
     uint32_t tot;
@@ -749,26 +739,19 @@ __global__ void __launch_bounds__(64) k_merge_long(SceneDev sc, const float2 *__
 
 size_t scan_tmp_elems_needed(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
-template <bool SWN>
-static void scan_impl(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
     size_t ntiles = (n + kScanTile - 1) / kScanTile;
     if (ntiles == 0) {
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tmp, (size_t)0, out + n);
         return;
     }
-    hipLaunchKernelGGL(k_scan_reduce<SWN>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, n, tmp);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tmp, ntiles, out + n);
-    hipLaunchKernelGGL(k_scan_down<SWN>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, out, n,
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, out, n,
                        (const uint32_t *)tmp);
 }
 
-void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
-    scan_impl<false>(in, out, n, tmp, s);
-}
 
-void scan_exclusive_swn(const uint32_t *meta, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s) {
-    scan_impl<true>(meta, out, n, tmp, s);
-}
 
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
     if (!n) return;

@@ -84,11 +84,10 @@ struct Frame {
     int2 *binfo;             // [NB] (path, row or -1)
     uint32_t *bcount;        // [NB+1] -> boff
     uint32_t *boff;          // [NB+1]
-    uint32_t *bwsum;         // [NB+1] -> bcarry
+    uint32_t *bwsum;         // [NB+1] per-bucket sum of scan windings -> bcarry
     uint32_t *bcarry;        // [NB+1]
     uint32_t *bcursor;       // [NB]
     uint32_t *frank;         // [F] rank of each fragment in its bucket (~0: none)
-    uint32_t *wscan;         // [F+1] exclusive scan of rec_swn
     uint32_t *rec_meta;      // [F] (valid fragments only are written)
     uint32_t *rec_mask;      // [F * words]
     uint32_t *scratch_keys;  // [F] sort keys of large buckets
@@ -137,8 +136,6 @@ struct Frame {
 // ---- launchers (sl_kernels.hip) ----
 size_t scan_tmp_elems_needed(size_t n);
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s);
-// exclusive scan of the sign-extended scan windings in fragment meta words (bits 16..23)
-void scan_exclusive_swn(const uint32_t *meta, uint32_t *out, size_t n, uint32_t *tmp, hipStream_t s);
 
 void launch_path_init(const SceneDev &sc, Frame &f, hipStream_t s);
 void launch_transform(const SceneDev &sc, Frame &f, hipStream_t s);
