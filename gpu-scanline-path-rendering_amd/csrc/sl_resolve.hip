@@ -384,9 +384,9 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
 
 constexpr int kResolveBlocks = 16384;
 
-// One small bucket's inputs, loaded a bucket ahead of its resolve (the grid-stride loop
-// keeps the next bucket's records and scalars in flight while the current one resolves;
-// offsets run two buckets ahead because the records' address depends on them).
+// One small bucket's records, loaded a bucket ahead of its resolve (the grid-stride loop
+// keeps the next bucket's records in flight while the current one resolves); the bucket
+// scalars come from per-lane registers (see k_resolve_first).
 template <int NW>
 struct SmallIn {
     uint32_t meta, mk[NW];
@@ -395,8 +395,7 @@ struct SmallIn {
 };
 
 template <int NW>
-__device__ __forceinline__ void small_load(const ResolveArgs &A, int64_t b, uint32_t base, uint32_t end, int2 info,
-                                           SmallIn<NW> &o) {
+__device__ __forceinline__ void small_load(const ResolveArgs &A, uint32_t base, uint32_t end, SmallIn<NW> &o) {
     const int lane = threadIdx.x & 63;
     const int n = (int)(end - base);
     o.meta = 0u;
@@ -405,14 +404,6 @@ __device__ __forceinline__ void small_load(const ResolveArgs &A, int64_t b, uint
     if (n > 0 && n <= 64 && lane < n) {
         o.meta = A.rec_meta[base + lane];
         load_mask<NW>(A.rec_mask, (size_t)base + lane, o.mk);
-    }
-    o.rule = A.frule[info.x];
-    o.carry = A.bcarry[b];
-    o.W = A.W;
-    if (A.pcanvas) {
-        const int4 cv = A.pcanvas[info.x];
-        o.W = cv.x;
-        o.carry -= A.scarry[cv.w];
     }
 }
 
@@ -492,48 +483,60 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
     __syncthreads();
 }
 
-// Every bucket: empty ones are cleared, buckets of at most 64 fragments are resolved,
+// Every bucket: empty ones are cleared, buckets of at most 128 fragments are resolved,
 // larger ones are listed for the size-class passes below.
 template <int SPP>
 __global__ void __launch_bounds__(64) k_resolve_first(ResolveArgs A) {
     constexpr int NW = Words<SPP>::n;
-    __shared__ uint32_t order[64], meta[64], msk[64 * NW];
+    __shared__ uint32_t key[128], order[128], meta[128], msk[128 * NW];
     const int lane = threadIdx.x;
     const int64_t G = gridDim.x;
-    int64_t b = blockIdx.x;
-    if (b >= A.NB) return;
-    auto head = [&](int64_t bb, uint32_t &base, uint32_t &end, int2 &info) {
-        if (bb < A.NB) {
-            base = A.boff[bb];
-            end = A.boff[bb + 1];
-            info = A.binfo[bb];
-        } else {
-            base = end = 0u;
-            info = make_int2(0, 0);
+    for (int64_t b0 = blockIdx.x; b0 < A.NB; b0 += 64 * G) {
+        // The wave's next 64 buckets b0 + k G: lane k loads their offsets and scalars with
+        // vector loads, read back per bucket by readlane.  (Scalar loads would share the
+        // LDS wait counter, so every LDS wait of a resolve would also wait for them.)
+        const int64_t bl = b0 + (int64_t)lane * G;
+        int vbase = 0, vend = 0, vcarry = 0, vrule = 0, vW = A.W;
+        if (bl < A.NB) {
+            const int2 info = A.binfo[bl];
+            vbase = (int)A.boff[bl];
+            // the out-of-canvas bucket holds records only for the winding carry
+            vend = info.y < 0 ? vbase : (int)A.boff[bl + 1];
+            vrule = A.frule[info.x];
+            uint32_t carry = A.bcarry[bl];
+            if (A.pcanvas) {
+                const int4 cv = A.pcanvas[info.x];
+                vW = cv.x;
+                carry -= A.scarry[cv.w];
+            }
+            vcarry = (int)carry;
         }
-    };
-    uint32_t base0, end0, base1, end1;
-    int2 info0, info1;
-    head(b, base0, end0, info0);
-    head(b + G, base1, end1, info1);
-    SmallIn<NW> in0, in1;
-    small_load<NW>(A, b, base0, end0, info0, in0);
-    for (; b < A.NB; b += G) {
-        uint32_t base2, end2;
-        int2 info2;
-        head(b + 2 * G, base2, end2, info2);
-        if (b + G < A.NB) small_load<NW>(A, b + G, base1, end1, info1, in1);
-        // the out-of-canvas bucket holds records only for the winding carry
-        const int n = info0.y < 0 ? 0 : (int)(end0 - base0);
-        if (n == 0) {
-            if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; A.btile[b] = 0ull; }
-        } else if (n > 64) {
-            if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)b;
-        } else {
-            resolve_small<SPP>(A, b, base0, n, in0, order, meta, msk);
+        const int64_t left = (A.NB - b0 + G - 1) / G;
+        const int nk = (int)(left < 64 ? left : 64);
+        SmallIn<NW> in0, in1;
+        small_load<NW>(A, (uint32_t)__builtin_amdgcn_readlane(vbase, 0), (uint32_t)__builtin_amdgcn_readlane(vend, 0),
+                       in0);
+        for (int k = 0; k < nk; ++k) {
+            const int64_t b = b0 + (int64_t)k * G;
+            const uint32_t base = (uint32_t)__builtin_amdgcn_readlane(vbase, k);
+            const int n = __builtin_amdgcn_readlane(vend, k) - (int)base;
+            if (k + 1 < nk)
+                small_load<NW>(A, (uint32_t)__builtin_amdgcn_readlane(vbase, k + 1),
+                               (uint32_t)__builtin_amdgcn_readlane(vend, k + 1), in1);
+            if (n == 0) {
+                if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; A.btile[b] = 0ull; }
+            } else if (n > 128) {
+                if (lane == 0) A.big_list[atomicAdd(A.big_count, 1u)] = (uint32_t)b;
+            } else if (n > 64) {
+                resolve_bucket<SPP, 2>(A, b, base, n, key, order, meta, msk);
+            } else {
+                in0.carry = (uint32_t)__builtin_amdgcn_readlane(vcarry, k);
+                in0.rule = __builtin_amdgcn_readlane(vrule, k);
+                in0.W = __builtin_amdgcn_readlane(vW, k);
+                resolve_small<SPP>(A, b, base, n, in0, order, meta, msk);
+            }
+            in0 = in1;
         }
-        base0 = base1; end0 = end1; info0 = info1; in0 = in1;
-        base1 = base2; end1 = end2; info1 = info2;
     }
 }
 
@@ -1429,7 +1432,6 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
 #define SL_LAUNCH(N)                                                                  \
     do {                                                                              \
         hipLaunchKernelGGL(k_resolve_first<N>, dim3(g1), b64, 0, s, A);               \
-        hipLaunchKernelGGL((k_resolve_class<N, 2>), dim3(gc), b64, 0, s, A);          \
         hipLaunchKernelGGL((k_resolve_class<N, 4>), dim3(gc), b64, 0, s, A);          \
         hipLaunchKernelGGL((k_resolve_class<N, 8>), dim3(gc), b64, 0, s, A);          \
         hipLaunchKernelGGL(k_resolve_huge<N>, dim3(1024), b64, 0, s, A);              \
