@@ -540,22 +540,26 @@ __global__ void __launch_bounds__(64) k_resolve_first(ResolveArgs A) {
     }
 }
 
-// Buckets of 64*S/2 < n <= 64*S fragments from the list.
-template <int SPP, int S>
-__global__ void __launch_bounds__(64) k_resolve_class(ResolveArgs A) {
+// The listed buckets (more than 128 fragments), one per block, in one launch so that the
+// few long ones overlap: up to 1024 fragments staged in LDS, more in k_resolve_huge.
+constexpr int kBigStage = 16;  // chunks of 64 fragments staged in LDS
+
+template <int SPP>
+__global__ void __launch_bounds__(64) k_resolve_big(ResolveArgs A) {
     constexpr int NW = Words<SPP>::n;
-    __shared__ uint32_t key[64 * S], order[64 * S], meta[64 * S], msk[64 * S * NW];
+    __shared__ uint32_t key[64 * kBigStage], order[64 * kBigStage], meta[64 * kBigStage], msk[64 * kBigStage * NW];
     const uint32_t nbig = *A.big_count;
     for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
         const int64_t b = A.big_list[bi];
         const uint32_t base = A.boff[b];
         const int n = (int)(A.boff[b + 1] - base);
-        if (n <= 32 * S || n > 64 * S) continue;
-        resolve_bucket<SPP, S>(A, b, base, n, key, order, meta, msk);
+        if (n <= 256) resolve_bucket<SPP, 4>(A, b, base, n, key, order, meta, msk);
+        else if (n <= 512) resolve_bucket<SPP, 8>(A, b, base, n, key, order, meta, msk);
+        else if (n <= 64 * kBigStage) resolve_bucket<SPP, kBigStage>(A, b, base, n, key, order, meta, msk);
     }
 }
 
-// Buckets of more than 512 fragments (rare): bitonic-sort (cx << 16 | index) keys in a
+// Buckets of more than 1024 fragments (rare): bitonic-sort (cx << 16 | index) keys in a
 // global scratch region, then walk the sorted order in chunks of 64 reading the records
 // from global memory.
 template <int SPP>
@@ -567,7 +571,7 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
         const int64_t b = A.big_list[bi];
         const uint32_t base = A.boff[b];
         const int n = (int)(A.boff[b + 1] - base);
-        if (n <= 512) continue;
+        if (n <= 64 * kBigStage) continue;
         if (n > 65536) {
             if (lane == 0) { atomicOr(A.err, 2u); A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; A.btile[b] = 0ull; }
             continue;
@@ -1167,12 +1171,21 @@ __global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
     for (int i = threadIdx.x; i < (SRGB ? 1024 : 0); i += 256) ((uint32_t *)s_lut)[i] = ((const uint32_t *)A.lut_c)[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // consecutive workgroups take consecutive tiles: the round-robin deal of workgroups to
-    // the 8 XCDs then spreads every band of rows over all XCDs, which balances the uneven
-    // per-row work better than giving each XCD a contiguous band (measured slower)
-    int tile = (int)blockIdx.x * 4 + w;
-    if (tile >= A.ntiles) return;
     int W = A.W, H = A.H, tiles_x = A.tiles_x, row0 = 0, pitch = A.pitch;
+    int tile;
+    if (A.nscenes == 0) {
+        // Workgroup b runs on XCD b % 8 (round-robin deal): XCD x takes the cell rows
+        // x, x + 8, x + 16, ... whole, its workgroups walking them tile by tile, so each
+        // row's items are fetched into one XCD's L2 and every XCD gets rows from the whole
+        // image (the per-row work is uneven, so contiguous bands would not balance).
+        const int xcd = (int)(blockIdx.x & 7u), t = (int)(blockIdx.x >> 3) * 4 + w;
+        const int rr = t / tiles_x, row = xcd + 8 * rr;
+        if (row >= (H + 1) / 2) return;
+        tile = row * tiles_x + (t - rr * tiles_x);
+    } else {
+        tile = (int)blockIdx.x * 4 + w;
+        if (tile >= A.ntiles) return;
+    }
     uint8_t *image = A.image;
     if (A.nscenes) {
         // the scene whose tile range holds this tile: last one with first tile <= tile
@@ -1432,8 +1445,7 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
 #define SL_LAUNCH(N)                                                                  \
     do {                                                                              \
         hipLaunchKernelGGL(k_resolve_first<N>, dim3(g1), b64, 0, s, A);               \
-        hipLaunchKernelGGL((k_resolve_class<N, 4>), dim3(gc), b64, 0, s, A);          \
-        hipLaunchKernelGGL((k_resolve_class<N, 8>), dim3(gc), b64, 0, s, A);          \
+        hipLaunchKernelGGL(k_resolve_big<N>, dim3(gc), b64, 0, s, A);                  \
         hipLaunchKernelGGL(k_resolve_huge<N>, dim3(1024), b64, 0, s, A);              \
     } while (0)
     if (t.spp == 8) SL_LAUNCH(8);
@@ -1500,7 +1512,8 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
         C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.step_c = t.step_c; C.lut_c = t.lut_c;
         C.image = f.image; C.pitch = f.pitch;
         C.nscenes = f.nscenes; C.ntiles = ntiles; C.stile = f.stile; C.simg = f.simg;
-        dim3 g(blocks_for(ntiles, 4)), b(256);
+        // one scene: 8 XCD row sets of ceil(R / 8) rows each; a batch: tiles in order
+        dim3 g(f.nscenes ? blocks_for(ntiles, 4) : 8u * blocks_for((int64_t)tiles_x * ((R + 7) / 8), 4)), b(256);
 #define SL_LAUNCH(N, SR)                                                                   \
     do {                                                                                   \
         if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4>), g, b, 0, s, C); \
