@@ -74,7 +74,7 @@ struct sl_handle {
     DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
         b_boff, b_bwsum, b_bcarry, b_bcursor, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
         b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
-        b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_image, b_scan_tmp;
+        b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_image, b_scan_tmp, b_lcount;
     // debug host copies
     std::vector<int32_t> d_ts, d_yx, d_pid, d_draw;
     std::vector<int8_t> d_wn;
@@ -279,6 +279,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         (rc = ensure(h, h->b_pcnt, (size_t)C * 20)) || (rc = ensure(h, h->b_nchain, ((size_t)C + 1) * 4)) ||
         (rc = ensure(h, h->b_choff, ((size_t)C + 1) * 4)) ||
         (rc = ensure(h, h->b_pbase, ((size_t)P + 1) * 4)) || (rc = ensure(h, h->b_small, 64)) ||
+        (rc = ensure(h, h->b_lcount, 1024)) ||
         (rc = ensure(h, h->b_image, image_bytes)) || (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
         return rc;
     f.tvtx = P_<float2>(h->b_tvtx); f.pflags = P_<uint32_t>(h->b_pflags); f.prow = P_<int2>(h->b_prow);
@@ -331,13 +332,13 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         h->last_F = 0; h->last_NCH = 0; h->last_NB = 0; h->sized = true;
         return SL_OK;
     }
-    if ((rc = ensure(h, h->b_ts, (size_t)F * 8)) || (rc = ensure(h, h->b_chains, (size_t)NCH * 18)) ||
+    if ((rc = ensure(h, h->b_ts, (size_t)F * 8)) || (rc = ensure(h, h->b_chains, (size_t)NCH * 6 + 16)) ||
         (rc = ensure(h, h->b_chain_t, (size_t)F * 4 + 64)))  // padded: read in aligned 16-byte groups
         return rc;
     f.ts = P_<int2>(h->b_ts);
     f.chains = P_<uint32_t>(h->b_chains);
-    f.plist = f.chains + 3 * (size_t)NCH;
-    f.lcount = P_<uint32_t>(h->b_small) + 8;
+    f.plist = f.chains + (size_t)NCH;
+    f.lcount = P_<uint32_t>(h->b_lcount);
     f.chain_t = P_<float>(h->b_chain_t);
     mark(2);
     sl::launch_intersect(sc, f, s);
@@ -535,7 +536,7 @@ int sl_destroy(sl_handle *h) {
                       &h->b_rec_mask, &h->b_scratch, &h->b_items, &h->b_imask, &h->b_icov, &h->b_bicnt, &h->b_bmcnt, &h->b_bext,
                       &h->b_big_list, &h->b_small, &h->b_nz_flag, &h->b_nz_off, &h->b_nzk0, &h->b_nzk1, &h->b_nzv0,
                       &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp,
-                      &h->b_rrec, &h->b_btile, &h->b_rtile, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
+                      &h->b_rrec, &h->b_btile, &h->b_rtile, &h->b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
     for (DevBuf *b : bufs) release(*b);
     if (h->events_ready)
         for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(h->ev[i]);

@@ -369,101 +369,137 @@ __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__r
     }
 }
 
-// Solve chains (curve << 4 | piece << 1 | side, non-empty only) and merge pieces
-// (curve << 3 | piece) listed in three length classes, longest first: the sequential
-// walks of long chains start early, and each wave's lanes carry similar lengths.
-__device__ __forceinline__ int len_class(uint32_t L) { return L >= 128 ? 0 : (L >= 16 ? 1 : 2); }
-
-__device__ __forceinline__ uint32_t wave_excl_scan_dpp(uint32_t v, uint32_t &total) {
-    // inclusive DPP scan (row_shr 1,2,4,8 + row_bcast 15/31), then exclusive by subtracting v
-    int x = (int)v;
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
-    total = (uint32_t)__builtin_amdgcn_readlane(x, 63);
-    return (uint32_t)x - v;
-}
-
+// Work lists.  Pieces (curve << 3 | piece) with nx + ny + 1 >= 128 cuts are "long": their
+// two solve chains (curve << 4 | piece << 1 | side, non-empty only) are listed for one
+// thread each, and the piece for the wave-parallel merge.  Shorter pieces are listed for
+// one thread that solves and walks both sides.  Chains and short pieces are grouped by
+// (cubic or closed-form solve, length bin), longest first, so the lanes of a wave run
+// loops of the same kind and about the same trip count and the long sequential walks
+// start early.  A count pass sizes the classes; the place pass puts every entry at its
+// class offset plus a rank taken per block in LDS and one reservation per class and block.
+constexpr int kLenBins = 13;
+constexpr int kChainClasses = 2 * kLenBins;                     // cubic bins, then closed-form bins
+constexpr int kLongPieceClass = kChainClasses;                  // one class
+constexpr int kShortClass0 = kChainClasses + 1;                 // cubic bins, then closed-form bins
+constexpr int kListClasses = kShortClass0 + 2 * kLenBins;
+constexpr uint32_t kLongPiece = 128;
 constexpr int kChainListThreads = 1024;
 
-__global__ void __launch_bounds__(kChainListThreads) k_chain_list(uint32_t C, const uint32_t *__restrict__ nchain,
-                                                    const uint32_t *__restrict__ pcnt, uint32_t cstride,
-                                                    uint32_t pstride, uint32_t *clist, uint32_t *plist,
-                                                    uint32_t *counts /* [3] chains, [3] pieces */) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    const bool live = c < C;
-    const int np = live ? (int)(nchain[c] >> 1) : 0;
+// length bins, longest first: >=128, 65-127, 33-64, 25-32, 17-24, 13-16, 9-12, 7-8, 5-6, 4, 3, 2, 1
+__device__ __forceinline__ int len_bin(uint32_t L) {
+    if (L >= 128) return 0;
+    if (L > 64) return 1;
+    if (L > 32) return 2;
+    if (L > 24) return 3;
+    if (L > 16) return 4;
+    if (L > 12) return 5;
+    if (L > 8) return 6;
+    if (L > 6) return 7;
+    if (L > 4) return 8;
+    return 13 - (int)L;
+}
+
+struct CurvePieces {
+    int np;
+    int cb;  // 0 for a cubic, kLenBins otherwise
     uint32_t pc[5];
+};
+
+__device__ __forceinline__ CurvePieces curve_pieces(const SceneDev &sc, uint32_t C, uint32_t c,
+                                                    const uint32_t *__restrict__ nchain,
+                                                    const uint32_t *__restrict__ pcnt) {
+    CurvePieces r;
+    const bool live = c < C;
+    r.np = live ? (int)(nchain[c] >> 1) : 0;
+    r.cb = (live && sc.ctype[c] == CT_CUBIC) ? 0 : kLenBins;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) pc[k] = k < np ? pcnt[5 * (size_t)c + k] : 0u;
-    // per-thread counts per class, one reservation per class and wave
-    uint32_t n[6] = {0u, 0u, 0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        if (k >= np) break;
-        const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
-        if (nx) n[len_class(nx)]++;
-        if (ny) n[len_class(ny)]++;
-        n[3 + len_class(nx + ny + 1)]++;
-    }
-    // wave scans, then one reservation per class and block (the six counters are shared by
-    // the whole grid, so fewer, larger reservations keep them from serialising)
-    __shared__ uint32_t s_off[kChainListThreads / 64][6];
-    const int wv = threadIdx.x >> 6;
-    uint32_t off[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        uint32_t tot;
-        off[q] = wave_excl_scan_dpp(n[q], tot);
-        if (lane == 0) s_off[wv][q] = tot;
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        const int q = (int)threadIdx.x;
-        uint32_t acc = 0;
-        for (int k = 0; k < kChainListThreads / 64; ++k) {
-            const uint32_t t = s_off[k][q];
-            s_off[k][q] = acc;
-            acc += t;
-        }
-        const uint32_t base = acc ? atomicAdd(&counts[q], acc) : 0u;
-        for (int k = 0; k < kChainListThreads / 64; ++k) s_off[k][q] += base;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 6; ++q) off[q] += s_off[wv][q];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        if (k >= np) break;
-        const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
-        if (nx) {
-            const int q = len_class(nx);
-            clist[(size_t)q * cstride + off[q]++] = (c << 4) | ((uint32_t)k << 1);
-        }
-        if (ny) {
-            const int q = len_class(ny);
-            clist[(size_t)q * cstride + off[q]++] = (c << 4) | ((uint32_t)k << 1) | 1u;
-        }
-        const int q = len_class(nx + ny + 1);
-        plist[(size_t)q * pstride + off[3 + q]++] = (c << 3) | (uint32_t)k;
+    for (int k = 0; k < 5; ++k) r.pc[k] = k < r.np ? pcnt[5 * (size_t)c + k] : 0u;
+    return r;
+}
+
+// the classes of piece k's entries: x chain, y chain (-1 when not listed), piece
+__device__ __forceinline__ void piece_classes(const CurvePieces &cp, int k, int &qx, int &qy, int &qp) {
+    const uint32_t nx = cp.pc[k] & 0xFFFFu, ny = cp.pc[k] >> 16, L = nx + ny + 1;
+    if (L >= kLongPiece) {
+        qx = nx ? cp.cb + len_bin(nx) : -1;
+        qy = ny ? cp.cb + len_bin(ny) : -1;
+        qp = kLongPieceClass;
+    } else {
+        qx = qy = -1;
+        qp = kShortClass0 + cp.cb + len_bin(L);
     }
 }
 
-// entry g of the class lists, in class order; false past their end
-__device__ __forceinline__ bool class_entry(uint32_t g, const uint32_t *__restrict__ list,
-                                            const uint32_t *__restrict__ cnt, uint32_t stride, uint32_t &d,
-                                            bool skip_first = false) {
-    const uint32_t n0 = skip_first ? 0u : cnt[0], n1 = cnt[1], n2 = cnt[2];
-    if (g < n0) { d = list[g]; return true; }
-    g -= n0;
-    if (g < n1) { d = list[stride + g]; return true; }
-    g -= n1;
-    if (g < n2) { d = list[2 * (size_t)stride + g]; return true; }
-    return false;
+// class counts (cnt[kListClasses])
+__global__ void __launch_bounds__(kChainListThreads) k_list_count(SceneDev sc, uint32_t C,
+                                                                  const uint32_t *__restrict__ nchain,
+                                                                  const uint32_t *__restrict__ pcnt, uint32_t *cnt) {
+    __shared__ uint32_t h[kListClasses];
+    for (int q = threadIdx.x; q < kListClasses; q += blockDim.x) h[q] = 0u;
+    __syncthreads();
+    const CurvePieces cp = curve_pieces(sc, C, blockIdx.x * blockDim.x + threadIdx.x, nchain, pcnt);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (k >= cp.np) break;
+        int qx, qy, qp;
+        piece_classes(cp, k, qx, qy, qp);
+        if (qx >= 0) atomicAdd(&h[qx], 1u);
+        if (qy >= 0) atomicAdd(&h[qy], 1u);
+        atomicAdd(&h[qp], 1u);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kListClasses; q += blockDim.x)
+        if (h[q]) atomicAdd(&cnt[q], h[q]);
+}
+
+// lists from the counts; cursor[kListClasses] starts at zero
+__global__ void __launch_bounds__(kChainListThreads) k_list_place(SceneDev sc, uint32_t C,
+                                                                  const uint32_t *__restrict__ nchain,
+                                                                  const uint32_t *__restrict__ pcnt,
+                                                                  const uint32_t *__restrict__ cnt, uint32_t *cursor,
+                                                                  uint32_t *clist, uint32_t *plist) {
+    __shared__ uint32_t h[kListClasses], base[kListClasses];
+    for (int q = threadIdx.x; q < kListClasses; q += blockDim.x) h[q] = 0u;
+    if (threadIdx.x == 0) {
+        // class offsets: within the chain list, and within the piece list (long pieces first)
+        uint32_t a = 0u;
+        for (int q = 0; q < kChainClasses; ++q) { base[q] = a; a += cnt[q]; }
+        a = 0u;
+        for (int q = kLongPieceClass; q < kListClasses; ++q) { base[q] = a; a += cnt[q]; }
+    }
+    __syncthreads();
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const CurvePieces cp = curve_pieces(sc, C, c, nchain, pcnt);
+    uint32_t rk[15];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (k >= cp.np) break;
+        int qx, qy, qp;
+        piece_classes(cp, k, qx, qy, qp);
+        rk[3 * k] = qx >= 0 ? atomicAdd(&h[qx], 1u) : 0u;
+        rk[3 * k + 1] = qy >= 0 ? atomicAdd(&h[qy], 1u) : 0u;
+        rk[3 * k + 2] = atomicAdd(&h[qp], 1u);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kListClasses; q += blockDim.x)
+        if (h[q]) base[q] += atomicAdd(&cursor[q], h[q]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (k >= cp.np) break;
+        int qx, qy, qp;
+        piece_classes(cp, k, qx, qy, qp);
+        if (qx >= 0) clist[base[qx] + rk[3 * k]] = (c << 4) | ((uint32_t)k << 1);
+        if (qy >= 0) clist[base[qy] + rk[3 * k + 1]] = (c << 4) | ((uint32_t)k << 1) | 1u;
+        plist[base[qp] + rk[3 * k + 2]] = (c << 3) | (uint32_t)k;
+    }
+}
+
+// total entries of classes [q0, q1)
+__device__ __forceinline__ uint32_t class_total(const uint32_t *__restrict__ cnt, int q0, int q1) {
+    uint32_t t = 0u;
+    for (int q = q0; q < q1; ++q) t += cnt[q];
+    return t;
 }
 
 // t | flags of a monotone piece's end: low bits 2 when the end lies on an integral x,
@@ -481,46 +517,66 @@ __device__ __forceinline__ float solve_cut(int type, float w, float a0, float a1
     if (type == CT_LINEAR) return tmin(tmax((cc - a0) * safe_rcp(a1 - a0), t_min), t1);
     if (type == CT_QUAD) return solve_quad_bezier(a0, a1, a2, cc, t_min, t1);
     if (type == CT_RATIONAL) return solve_quad_bezier(a0 - cc, (a1 - cc) * w, a2 - cc, 0.f, t_min, t1);
+    // 24 bisection steps.  lo moves only to a midpoint whose f - cc has the sign bit of
+    // f(lo) - cc, so that sign bit stays the one of f(t_min) - cc.
     float lo = t_min, hi = t1;
-    float vlo = curve_eval(CT_CUBIC, 0.f, a0, a1, a2, a3, lo);
+    const float vlo = curve_eval(CT_CUBIC, 0.f, a0, a1, a2, a3, lo);
     float t_solve = lo;
     if (vlo != cc) {
+        const uint32_t s0 = fb(vlo - cc) >> 31;
         float last_v = 0.f;
         for (int it = 0; it < 24; ++it) {
-            float tm = (lo + hi) * .5f;
-            float vm = curve_eval(CT_CUBIC, 0.f, a0, a1, a2, a3, tm);
+            const float tm = (lo + hi) * .5f;
+            const float vm = curve_eval(CT_CUBIC, 0.f, a0, a1, a2, a3, tm);
+            const bool up = (fb(vm - cc) >> 31) == s0;
+            lo = up ? tm : lo;
+            hi = up ? hi : tm;
             t_solve = tm;
             last_v = vm;
-            if ((int32_t)(fb(vm - cc) ^ fb(vlo - cc)) >= 0) { lo = tm; vlo = vm; }
-            else { hi = tm; }
         }
         if (fabsf(last_v - cc) > 1.f) t_solve = t_min;
     }
     return t_solve;
 }
 
-// Intersection pass 1: one thread per (curve, piece, side) chain.  In the reference's
-// merge walk (ras_cut.h:910-1232) the j-th cut on one side is solved from the t the same
-// side produced last (or the piece start), so each side's chain is independent of the
-// other side's: solve it sequentially, storing t | side for the merge pass.
-__global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
-                                                 const uint8_t *__restrict__ cncut, const uint32_t *__restrict__ coff,
-                                                 const uint32_t *__restrict__ pcnt, const uint32_t *__restrict__ clist,
-                                                 const uint32_t *__restrict__ counts, uint32_t cstride, int W, int H,
-                                                 float *chain_t) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t d;
-    if (!class_entry(i, clist, counts, cstride, d)) return;
-    const uint32_t c = d >> 4;
-    const int k = (int)((d >> 1) & 7u), side = (int)(d & 1u);
+// The cut solve state of one side of a monotone piece.  In the reference's merge walk
+// (ras_cut.h:910-1232) the j-th cut on one side is solved from the t the same side
+// produced last (or the piece start), so each side's chain is independent of the other
+// side's and is solved in order, each t stored as t | side.
+struct SideSolver {
+    float a0, a1, a2, a3, cc, dd, t_min, t1m, w;
+    int type;
+    uint32_t side;
+    __device__ __forceinline__ float next() {
+        const float t = solve_cut(type, w, a0, a1, a2, a3, cc, t_min, t1m);
+        const float tm = bf((fb(t) & 0xFFFFFFFCu) | side);
+        t_min = tm;
+        cc += dd;
+        return tm;
+    }
+};
+
+struct PieceSetup {
+    uint32_t c, base, nx, ny;
+    float t0m;
+    SideSolver sx, sy;
+};
+
+// piece k of curve c: its ts offset, cut counts, start flags and both sides' solvers
+__device__ __forceinline__ PieceSetup piece_setup(const SceneDev &sc, const float2 *__restrict__ tv,
+                                                  const float4 *__restrict__ ccuts, const uint8_t *__restrict__ cncut,
+                                                  const uint32_t *__restrict__ coff, const uint32_t *__restrict__ pcnt,
+                                                  uint32_t c, int k, int W, int H) {
+    PieceSetup p;
+    p.c = c;
     const uint32_t *pc = pcnt + 5 * (size_t)c;
     uint32_t off = coff[c];
     for (int kk = 0; kk < k; ++kk) off += 1u + (pc[kk] & 0xFFFFu) + (pc[kk] >> 16);
-    const uint32_t nx = pc[k] & 0xFFFFu, ny = pc[k] >> 16;
-    const uint32_t cnt = side ? ny : nx;
-    if (cnt == 0) return;
+    p.base = off;
+    p.nx = pc[k] & 0xFFFFu;
+    p.ny = pc[k] >> 16;
     path_canvas(sc, sc.cpath[c], W, H);
-    Curve cu = load_curve(sc, tv, c);
+    const Curve cu = load_curve(sc, tv, c);
     const float4 q = ccuts[c];
     const int n = cncut[c];
     float v0x = cu.x0, v0y = cu.y0, t0m = 0.f;
@@ -530,22 +586,42 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
         v0y = ey(cu, tr);
         t0m = piece_end_flags(tr, v0x);
     }
+    p.t0m = t0m;
     const float t1r = (k == n) ? 1.f : pick5(k, q.x, q.y, q.z, q.w, 1.f);
     const float v1x = ex(cu, t1r), v1y = ey(cu, t1r);
     const float t1m = piece_end_flags(t1r, v1x);
     const CutRange r = cut_range(v0x, v0y, v1x, v1y, W, H);
-    float cc = side ? (float)(r.dy < 0.f ? r.ye : r.yb) : (float)(r.dx < 0.f ? r.xe : r.xb);
-    const float dd = side ? r.dy : r.dx;
-    const float a0 = side ? cu.y0 : cu.x0, a1 = side ? cu.y1 : cu.x1;
-    const float a2 = side ? cu.y2 : cu.x2, a3 = side ? cu.y3 : cu.x3;
-    float *out = chain_t + off + (side ? nx : 0u);
+    p.sx.a0 = cu.x0; p.sx.a1 = cu.x1; p.sx.a2 = cu.x2; p.sx.a3 = cu.x3;
+    p.sx.cc = (float)(r.dx < 0.f ? r.xe : r.xb);
+    p.sx.dd = r.dx;
+    p.sy.a0 = cu.y0; p.sy.a1 = cu.y1; p.sy.a2 = cu.y2; p.sy.a3 = cu.y3;
+    p.sy.cc = (float)(r.dy < 0.f ? r.ye : r.yb);
+    p.sy.dd = r.dy;
+    p.sx.t_min = p.sy.t_min = t0m;
+    p.sx.t1m = p.sy.t1m = t1m;
+    p.sx.w = p.sy.w = cu.w;
+    p.sx.type = p.sy.type = cu.type;
+    p.sx.side = 0u;
+    p.sy.side = 1u;
+    return p;
+}
+
+// Intersection pass 1, long pieces: one thread per (curve, piece, side) chain, the t's
+// stored for the merge pass (k_merge_long) with 16-byte stores.
+__device__ __forceinline__ void solve_chain(const SceneDev &sc, const float2 *__restrict__ tv,
+                                            const float4 *__restrict__ ccuts, const uint8_t *__restrict__ cncut,
+                                            const uint32_t *__restrict__ coff, const uint32_t *__restrict__ pcnt,
+                                            uint32_t d, int W, int H, float *chain_t) {
+    const int side = (int)(d & 1u);
+    PieceSetup p = piece_setup(sc, tv, ccuts, cncut, coff, pcnt, d >> 4, (int)((d >> 1) & 7u), W, H);
+    const uint32_t cnt = side ? p.ny : p.nx;
+    SideSolver sv = side ? p.sy : p.sx;
+    float *out = chain_t + p.base + (side ? p.nx : 0u);
     // scalar stores up to a 16-byte boundary, then whole float4 stores, then the tail
     const uint32_t head = (uint32_t)((4u - (((uintptr_t)out >> 2) & 3u)) & 3u);
-    float t_min = t0m;
     float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t j = 0; j < cnt; ++j) {
-        const float t = solve_cut(cu.type, cu.w, a0, a1, a2, a3, cc, t_min, t1m);
-        const float tm = bf((fb(t) & 0xFFFFFFFCu) | (uint32_t)side);
+        const float tm = sv.next();
         if (j < head) {
             out[j] = tm;
         } else {
@@ -556,8 +632,6 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
             else q4.w = tm;
             if (ph == 3) *(float4 *)(out + j - 3) = q4;
         }
-        t_min = tm;
-        cc += dd;
     }
     if (cnt > head) {
         const uint32_t rem = (cnt - head) & 3u;  // values still held in q4
@@ -656,15 +730,58 @@ __device__ __forceinline__ void walk_piece(const PieceRef &r, const float *__res
     }
 }
 
-// Short and medium pieces: one thread per piece walks it.
-__global__ void __launch_bounds__(256) k_merge(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
-                                                const uint32_t *__restrict__ coff, const uint32_t *__restrict__ pcnt,
-                                                const float *__restrict__ chain_t, const uint32_t *__restrict__ plist,
-                                                const uint32_t *__restrict__ counts, uint32_t pstride, int2 *ts) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t d;
-    if (!class_entry(g, plist, counts, pstride, d, true)) return;
-    walk_piece(piece_ref(sc, tv, ccuts, coff, pcnt, d), chain_t, ts);
+// Short and medium pieces (nx + ny + 1 < 128): one thread solves both sides and replays
+// the slot walk as it goes, each slot refilled by the next cut of its side -- the same
+// sequence walk_piece reads from the stored chains, without storing them.
+__device__ __forceinline__ void walk_piece_solving(PieceSetup &p, int2 *ts) {
+    int2 *out = ts + p.base;
+    const uint32_t nx = p.nx, ny = p.ny;
+    const float two = 2.f, two1 = bf(fb(2.f) | 1u);
+    float xs, ys;
+    uint32_t rx, ry;
+    if (nx == 0) { xs = two; ys = p.t0m; rx = 0; ry = ny; }
+    else if (ny == 0) { xs = p.t0m; ys = two1; rx = nx; ry = 0; }
+    else { xs = p.sx.next(); rx = nx - 1; ys = p.t0m; ry = ny; }
+    uint32_t last = fb(-1.f);
+    const uint32_t n_loop = nx + ny + 1;
+    for (uint32_t it = 0; it < n_loop; ++it) {
+        float v;
+        if (xs <= ys) {
+            v = xs;
+            if (rx > 0) { xs = p.sx.next(); rx--; } else { xs = two; }
+        } else {
+            v = ys;
+            if (ry > 0) { ys = p.sy.next(); ry--; } else { ys = two1; }
+        }
+        uint32_t o = fb(v);
+        if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
+            o |= last;
+            out[it - 1].y = (int)o;
+        }
+        out[it] = make_int2((int)p.c, (int)o);
+        last = o;
+    }
+}
+
+// Intersection pass 1.  Blocks [0, chain_blocks) solve the listed chains of long pieces,
+// longest first; the blocks after them each take 256 short pieces and walk them while
+// solving, so the short pieces fill the GPU while the long chains run.
+__global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
+                                                 const uint8_t *__restrict__ cncut, const uint32_t *__restrict__ coff,
+                                                 const uint32_t *__restrict__ pcnt, const uint32_t *__restrict__ clist,
+                                                 const uint32_t *__restrict__ plist, const uint32_t *__restrict__ cnt,
+                                                 uint32_t chain_blocks, int W, int H, float *chain_t, int2 *ts) {
+    if (blockIdx.x < chain_blocks) {
+        const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= class_total(cnt, 0, kChainClasses)) return;
+        solve_chain(sc, tv, ccuts, cncut, coff, pcnt, clist[i], W, H, chain_t);
+        return;
+    }
+    const uint32_t g = (blockIdx.x - chain_blocks) * blockDim.x + threadIdx.x;
+    if (g >= class_total(cnt, kShortClass0, kListClasses)) return;
+    const uint32_t d = plist[cnt[kLongPieceClass] + g];
+    PieceSetup p = piece_setup(sc, tv, ccuts, cncut, coff, pcnt, d >> 3, (int)(d & 7u), W, H);
+    walk_piece_solving(p, ts);
 }
 
 // Long pieces: one wavefront per piece merges in parallel.  The x chain X and the y
@@ -783,25 +900,24 @@ void launch_monotone(const SceneDev &sc, Frame &f, hipStream_t s) {
 
 void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.C) return;
-    const uint32_t cstride = (uint32_t)f.NCH, pstride = (uint32_t)(f.NCH / 2);
-    launch_fill_u32(f.lcount, 0u, 6, s);
-    hipLaunchKernelGGL(k_chain_list, dim3(blocks_for(sc.C, kChainListThreads)), dim3(kChainListThreads), 0, s, sc.C,
-                       (const uint32_t *)f.nchain,
-                       (const uint32_t *)f.pcnt, cstride, pstride, f.chains, f.plist, f.lcount);
+    uint32_t *cnt = f.lcount, *cursor = f.lcount + kListClasses;
+    launch_fill_u32(f.lcount, 0u, 2 * kListClasses, s);
+    const dim3 gl(blocks_for(sc.C, kChainListThreads)), bl(kChainListThreads);
+    hipLaunchKernelGGL(k_list_count, gl, bl, 0, s, sc, sc.C, (const uint32_t *)f.nchain, (const uint32_t *)f.pcnt, cnt);
+    hipLaunchKernelGGL(k_list_place, gl, bl, 0, s, sc, sc.C, (const uint32_t *)f.nchain, (const uint32_t *)f.pcnt,
+                       (const uint32_t *)cnt, cursor, f.chains, f.plist);
+    // chain blocks: enough for every chain; piece blocks: enough for every piece
+    const uint32_t npieces = (uint32_t)(f.NCH / 2);
+    const uint32_t chain_blocks = blocks_for(f.NCH, 256), piece_blocks = blocks_for(npieces, 256);
     if (f.NCH)
-        hipLaunchKernelGGL(k_chains, dim3(blocks_for(f.NCH, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
+        hipLaunchKernelGGL(k_chains, dim3(chain_blocks + piece_blocks), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
                            (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
-                           (const uint32_t *)f.pcnt, (const uint32_t *)f.chains, (const uint32_t *)f.lcount, cstride,
-                           f.W, f.H, f.chain_t);
-    if (pstride) {
+                           (const uint32_t *)f.pcnt, (const uint32_t *)f.chains, (const uint32_t *)f.plist,
+                           (const uint32_t *)cnt, chain_blocks, f.W, f.H, f.chain_t, f.ts);
+    if (npieces)
         hipLaunchKernelGGL(k_merge_long, dim3(4096), dim3(64), 0, s, sc, (const float2 *)f.tvtx, (const float4 *)f.ccuts,
                            (const uint32_t *)f.coff, (const uint32_t *)f.pcnt, (const float *)f.chain_t,
-                           (const uint32_t *)f.plist, (const uint32_t *)(f.lcount + 3), f.ts);
-        hipLaunchKernelGGL(k_merge, dim3(blocks_for(pstride, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                           (const float4 *)f.ccuts, (const uint32_t *)f.coff, (const uint32_t *)f.pcnt,
-                           (const float *)f.chain_t, (const uint32_t *)f.plist, (const uint32_t *)(f.lcount + 3),
-                           pstride, f.ts);
-    }
+                           (const uint32_t *)f.plist, (const uint32_t *)(cnt + kLongPieceClass), f.ts);
 }
 
 }  // namespace sl

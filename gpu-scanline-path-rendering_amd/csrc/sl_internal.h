@@ -70,9 +70,10 @@ struct Frame {
     uint32_t *pcnt;          // [5C] per monotone piece: x cuts | y cuts << 16
     uint32_t *nchain;        // [C+1] solve chains per curve (2 per piece)
     uint32_t *choff;         // [C+1] exclusive scan of nchain
-    uint32_t *chains;        // [3][NCH] solve chain descriptors in length classes
-    uint32_t *plist;         // [3][NCH/2] merge piece descriptors in length classes
-    uint32_t *lcount;        // [6] class list lengths: chains, then pieces
+    uint32_t *chains;        // [NCH] solve chains of long pieces, grouped by class
+    uint32_t *plist;         // [NCH/2] pieces: long ones, then short ones grouped by class
+    uint32_t *lcount;        // [2 * 53] class counts (26 chain, 1 long-piece, 26 short-piece
+                             //   classes), then the place pass's cursors
     int64_t NCH;
     float *chain_t;          // [F] solved t | side per chain
     // fragments
