@@ -820,23 +820,31 @@ __global__ void __launch_bounds__(64) k_merge_long(SceneDev sc, const float2 *__
         __syncthreads();
         // running-maximum key of B[j]: max(t0m, Y[j-1]) for j >= 1 (Y is sorted)
         auto bkey = [&](uint32_t j) { return j == 0 ? sb[0] : fmaxf(sb[0], sb[j]); };
-        for (uint32_t i = lane; i < nx; i += 64) {
-            const float x = sa[i];
-            uint32_t lo = 0, hi = ny + 1;  // #{j : key(B[j]) < x}
+        // merge path: each lane merges one contiguous run of outputs [p0, p1), starting
+        // from the number of X entries among the first p0 outputs (the first i with
+        // X[i] > key(B[p0 - i - 1]), found by binary search)
+        const uint32_t nb = ny + 1, seg = (n + 63) >> 6;
+        const uint32_t p0 = tmin(n, (uint32_t)lane * seg), p1 = tmin(n, p0 + seg);
+        if (p0 < p1) {
+            uint32_t lo = p0 > nb ? p0 - nb : 0u, hi = tmin(p0, nx);
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (bkey(mid) < x) lo = mid + 1; else hi = mid;
+                if (sa[mid] <= bkey(p0 - mid - 1)) lo = mid + 1; else hi = mid;
             }
-            sm[i + lo] = fb(x);
-        }
-        for (uint32_t j = lane; j <= ny; j += 64) {
-            const float key = bkey(j);
-            uint32_t lo = 0, hi = nx;  // #{i : X[i] <= key}
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sa[mid] <= key) lo = mid + 1; else hi = mid;
+            uint32_t i = lo, j = p0 - lo;
+            float xv = i < nx ? sa[i] : 0.f, bk = j < nb ? bkey(j) : 0.f;
+            for (uint32_t p = p0; p < p1; ++p) {
+                const bool takex = i < nx && (j >= nb || xv <= bk);
+                if (takex) {
+                    sm[p] = fb(xv);
+                    ++i;
+                    if (i < nx) xv = sa[i];
+                } else {
+                    sm[p] = fb(sb[j]);
+                    ++j;
+                    if (j < nb) bk = bkey(j);
+                }
             }
-            sm[j + lo] = fb(sb[j]);
         }
         }
         __syncthreads();
