@@ -789,7 +789,7 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
 // by flag bits alone; with B ordered by its running maximum the walk is a stable merge
 // (x first on ties): X[i] lands at i + #{B keys < X[i]}, B[j] at j + #{X <= key(B[j])}.
 // Runs of equal t then take the OR of their flags up to the next entry of the run.
-constexpr int kMergeCap = 2048;
+constexpr int kMergeCap = 1024;
 
 __global__ void __launch_bounds__(64) k_merge_long(SceneDev sc, const float2 *__restrict__ tv,
                                                    const float4 *__restrict__ ccuts, const uint32_t *__restrict__ coff,
