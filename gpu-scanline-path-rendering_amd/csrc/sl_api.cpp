@@ -59,7 +59,7 @@ struct sl_handle {
     int64_t batch_rows = 0, batch_tiles = 0, batch_image_bytes = 0;
     DevBuf pcanvas, stile, simg, sfirst, scarry;
     // launch shape (results do not depend on it)
-    int resolve_grid = 0, class_grid = 0, comp_group = 8, frag_block = 256;
+    int resolve_grid = 0, class_grid = 0, comp_group = 4, frag_block = 256;
     // sizes of the last eager render; a replay of the same scene reuses them
     bool sized = false;
     uint32_t last_F = 0, last_NCH = 0, last_NB = 0;
@@ -808,7 +808,7 @@ struct sl_batch {
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int shape[4] = {0, 0, 8, 256};        // resolve grid, class grid, compositor group, fragment block
+    int shape[4] = {0, 0, 4, 256};        // resolve grid, class grid, compositor group, fragment block
     std::string err;
 };
 

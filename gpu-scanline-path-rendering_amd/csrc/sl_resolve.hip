@@ -1134,7 +1134,7 @@ struct CovArgs {
     const uint64_t *simg;
 };
 
-constexpr int kCovList = 128;   // per-wave list of the tile's buckets
+constexpr int kCovList = 64;    // per-wave list of the tile's buckets
 
 // Scatter the starts of a long bucket's items (more than 64) into the column slots.
 __device__ __noinline__ void scatter_big(const uint32_t *items, uint32_t ib, int cnt, int tx0, uint32_t *slot) {
@@ -1156,8 +1156,10 @@ __device__ __noinline__ void scatter_big(const uint32_t *items, uint32_t ib, int
 // (items of a bucket are disjoint and sorted by x), an inclusive max-scan over the lanes
 // then names each column's covering candidate, and the lanes blend in path order:
 // coverage alpha = paint alpha * covered samples / N (output_{8,32}.frag.glsl).
+// The kernel is bound by VALU issue, LDS and load latency together; eight waves per SIMD
+// (registers capped at 64, a few spilled) hide the latency best.
 template <int SPP, bool SRGB, int kCovGroup>
-__global__ void __launch_bounds__(256) k_composite_cov(CovArgs A) {
+__global__ void __launch_bounds__(256, 8) k_composite_cov(CovArgs A) {
     __shared__ float4 s_step[SRGB ? 256 : 1];
     __shared__ uint8_t s_lut[SRGB ? 4096 : 4];
     __shared__ float s_dec[256];
