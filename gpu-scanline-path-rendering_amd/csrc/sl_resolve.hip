@@ -486,7 +486,7 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
 // Every bucket: empty ones are cleared, buckets of at most 128 fragments are resolved,
 // larger ones are listed for the size-class passes below.
 template <int SPP>
-__global__ void __launch_bounds__(64) k_resolve_first(ResolveArgs A) {
+__global__ void __launch_bounds__(64, 6) k_resolve_first(ResolveArgs A) {
     constexpr int NW = Words<SPP>::n;
     __shared__ uint32_t key[128], order[128], meta[128], msk[128 * NW];
     const int lane = threadIdx.x;
