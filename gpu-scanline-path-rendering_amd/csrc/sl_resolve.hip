@@ -1277,16 +1277,10 @@ __global__ void __launch_bounds__(256, 8) k_composite_cov(CovArgs A) {
     const uint32_t q0 = A.rowbeg[row0 + r], q1 = A.rowend[row0 + r];
     const int tbit = (tx0 >> 6) & 63;
     int nlist = 0;
-    uint4 nxt = (q0 + lane < q1) ? A.rrec[q0 + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
-    uint64_t nxt_t = (q0 + lane < q1) ? A.rtile[q0 + lane] : 0ull;
     for (uint32_t qc = q0; qc < q1; qc += 64) {
-        const uint4 rec = nxt;
-        const uint64_t rt = nxt_t;
         const bool inq = qc + lane < q1;
-        if (qc + 64 < q1) {
-            nxt = (qc + 64 + lane < q1) ? A.rrec[qc + 64 + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
-            nxt_t = (qc + 64 + lane < q1) ? A.rtile[qc + 64 + lane] : 0ull;
-        }
+        const uint4 rec = inq ? A.rrec[qc + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
+        const uint64_t rt = inq ? A.rtile[qc + lane] : 0ull;
         const int exlo = (int)(rec.x & 0xFFFFu), exhi = (int)(rec.x >> 16);
         const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tbit) & 1ull);
         const unsigned long long om = __ballot(ov);
