@@ -24,8 +24,10 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_bytes(stage, cnt, spp, W, H):
-    """Algorithmic HBM bytes of one stage's launch (each datum read or written once)."""
+def stage_bytes(stage, cnt, spp, W, H, ms=False):
+    """Algorithmic HBM bytes of one stage's launch (each datum read or written once).  The
+    compositor's per-bucket row records (24 B per non-empty bucket, under 10 % of its bytes)
+    are not counted."""
     m = spp // 2  # mask bytes per 2x2 cell
     F, M, S = cnt["curve_fragment"], cnt["merged_fragment"], cnt["span"]
     C, V, P = cnt["curve"], cnt["vertex"], cnt["path"]
@@ -34,15 +36,37 @@ def stage_bytes(stage, cnt, spp, W, H):
         return 30 * C + 8 * V + 8 * F
     if stage == "fragments":      # ts in; per fragment record out (meta 4 + mask m)
         return 8 * F + (4 + m) * F
-    if stage == "resolve":        # records in; items (4 B) + resolved masks (m) out
-        return (4 + m) * F + 4 * items + m * M
-    if stage == "composite":      # items + resolved masks in; RGBA8 image out
-        return 4 * items + m * M + 4 * W * H
+    if stage == "resolve":        # records in; items (4 B) + per-pixel counts (coverage) or masks out
+        return (4 + m) * F + 4 * items + (m if ms else 4) * M
+    if stage == "composite":      # items + per-pixel sample counts (coverage) or masks in; RGBA8 out
+        return 4 * items + (m if ms else 4) * M + 4 * W * H
     if stage == "transform_cull":
         return 8 * V + 4 * V + 8 * V + 4 * P
     if stage == "monotone_count":
         return 30 * C + 8 * V + 21 * C
     return 0
+
+
+# the kernel that makes up each stage's time (the stage's HIP-event interval)
+STAGE_KERNEL = {("composite", False): "k_composite_cov", ("composite", True): "k_composite",
+                ("resolve", False): "k_resolve_first", ("resolve", True): "k_resolve_first",
+                ("fragments", False): "k_frag_scatter", ("fragments", True): "k_frag_scatter",
+                ("intersect", False): "k_chains", ("intersect", True): "k_chains"}
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plain_2048_32spp_traffic.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the headline workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("kernel_traffic_bytes:"):
+                    return json.loads(line.split(":", 1)[1]).get(kernel)
+    except OSError:
+        pass
+    return None
 
 
 def b_alg_total(cnt, spp, W, H):
@@ -314,9 +338,13 @@ def main():
 
     F = cnt["curve_fragment"]
     value = F / (ms_step * 1e-3) / 1e6
-    dom = max((k for k in stages if stage_bytes(k, cnt, args.spp, W, H) > 0), key=lambda k: stages[k])
-    dom_bytes = stage_bytes(dom, cnt, args.spp, W, H)
+    dom = max((k for k in stages if stage_bytes(k, cnt, args.spp, W, H, args.ms) > 0), key=lambda k: stages[k])
+    dom_bytes = stage_bytes(dom, cnt, args.spp, W, H, args.ms)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
+    dom_kernel = STAGE_KERNEL.get((dom, args.ms), dom)
+    traffic = None
+    if args.config == 2 and not args.ms and (W, args.paths, args.spp) == (2048, 20000, 32):
+        traffic = profiled_traffic(dom_kernel)
     b_alg = b_alg_total(cnt, args.spp, W, H)
 
     cpu = None
@@ -344,9 +372,9 @@ def main():
         "b_alg_bytes": int(b_alg),
         "whole_frame_gbs": round(b_alg / (ms_step * 1e-3) / 1e9, 2),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None},
+                     "alg_bytes": int(dom_bytes), "traffic": traffic},
         "cpu_baseline": cpu,
     }
     if cpu:

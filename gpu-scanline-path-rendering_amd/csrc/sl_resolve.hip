@@ -1490,6 +1490,11 @@ void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_row_bounds, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, (const uint32_t *)f.nz_keys[0], n_ptr,
                        f.rowbeg, f.rowend);
+    if (!f.ms)
+        hipLaunchKernelGGL(k_row_records, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB, n_ptr,
+                           (const uint32_t *)f.nz_vals[0], (const uint32_t *)f.bext, (const int2 *)f.binfo, sc.finfo,
+                           (const uint32_t *)f.bicnt, (const uint32_t *)f.boff, (const uint64_t *)f.btile, f.rrec,
+                           f.rtile);
 }
 
 void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {
@@ -1497,11 +1502,6 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
     const int tiles_x = (f.W + 63) / 64;
     const int64_t ntiles = f.nscenes ? f.ntiles : (int64_t)R * tiles_x;
     if (!f.ms) {
-        if (f.NB)
-            hipLaunchKernelGGL(k_row_records, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB,
-                               (const uint32_t *)(f.nz_off + f.NB), (const uint32_t *)f.nz_vals[0],
-                               (const uint32_t *)f.bext, (const int2 *)f.binfo, sc.finfo, (const uint32_t *)f.bicnt,
-                               (const uint32_t *)f.boff, (const uint64_t *)f.btile, f.rrec, f.rtile);
         CovArgs C;
         C.W = f.W; C.H = f.H; C.tiles_x = tiles_x;
         C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec; C.rtile = f.rtile;
