@@ -226,6 +226,18 @@ __device__ __forceinline__ int dpp_scan(int x, int ident, Op op) {
     x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
     return x;
 }
+// The same scan inside segments of S lanes: S = 16 keeps to the DPP rows, S = 32 adds the
+// row_bcast:15 step (rows 1 and 3 take the last lane of rows 0 and 2), S = 64 is dpp_scan.
+template <int S, class Op>
+__device__ __forceinline__ int dpp_scan_seg(int x, int ident, Op op) {
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    if (S >= 32) x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    if (S >= 64) x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
 struct OpAdd { __device__ int operator()(int a, int b) const { return a + b; } };
 struct OpMin { __device__ int operator()(int a, int b) const { return a < b ? a : b; } };
 struct OpMax { __device__ int operator()(int a, int b) const { return a > b ? a : b; } };
@@ -483,6 +495,101 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
     __syncthreads();
 }
 
+// Buckets of at most S (16 or 32) fragments, 64 / S of them in one wave: lane l works on
+// segment l / S, whose records, rank sort, scans and reductions stay inside its S lanes
+// (segmented DPP scans), so one pass resolves the whole pack.  ks[s] is the s-th bucket's
+// slot in the per-lane scalars (lane k holds bucket b0 + k G), -1 for an empty segment.
+template <int SPP, int S>
+__device__ __forceinline__ void resolve_pack(const ResolveArgs &A, int64_t b0, int64_t G, const int *ks, int vbase,
+                                             int vend, int vcarry, int vrule, int vW, uint32_t *order, uint32_t *meta,
+                                             uint32_t *msk) {
+    constexpr int NW = Words<SPP>::n;
+    constexpr int P = 64 / S;
+    const int lane = threadIdx.x & 63;
+    const int seg = lane / S, i = lane % S, sb = seg * S;
+    int k = -1;
+#pragma unroll
+    for (int s = 0; s < P; ++s)
+        if (seg == s) k = ks[s];
+    // every lane takes part in the cross-lane reads (a lane outside the exec mask would
+    // supply zeros to the lanes reading from it)
+    const int src = k < 0 ? 0 : k;
+    const uint32_t base = (uint32_t)__shfl(vbase, src, 64);
+    const int end = __shfl(vend, src, 64);
+    const uint32_t wcarry = (uint32_t)__shfl(vcarry, src, 64);
+    const int rule = __shfl(vrule, src, 64);
+    const int W = __shfl(vW, src, 64);
+    const int n = k < 0 ? 0 : end - (int)base;
+    uint32_t mykey = 0xFFFFFFFFu;
+    if (i < n) {
+        const uint32_t m = A.rec_meta[base + i];
+        uint32_t mk[NW];
+        load_mask<NW>(A.rec_mask, (size_t)base + i, mk);
+        meta[lane] = m;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) msk[lane * NW + q] = mk[q];
+        mykey = ((m & 0xFFFFu) << 10) | (uint32_t)i;
+    }
+    int rk = 0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) rk += ((uint32_t)__shfl((int)mykey, sb + j, 64) < mykey) ? 1 : 0;
+    if (i < n) order[sb + rk] = (uint32_t)lane;  // LDS index of the record
+    __syncthreads();
+
+    const int p = i;
+    const bool v = p < n;
+    const uint32_t m = v ? meta[order[sb + p]] : 0u;
+    const int cx = (int)(m & 0xFFFFu);
+    const int swn = v ? (int)(int8_t)(m >> 16) : 0;
+    const int prev_cx = (v && p > 0) ? (int)(meta[order[sb + p - 1]] & 0xFFFFu) : -1;
+    const bool head = v && (p == 0 || cx != prev_cx);
+    const int incl = dpp_scan_seg<S>(swn, 0, OpAdd());
+    const int8_t w_at = (int8_t)(int)(wcarry + (uint32_t)(incl - swn));
+    const int x = 2 * cx - 2;
+    const bool merged = head && x >= 0 && x < W;
+    // the group of a head lane ends at the next head of its segment (or the bucket end)
+    const uint64_t heads = __ballot(head);
+    uint32_t res[NW];
+    if (merged) {
+        const uint64_t segm = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << sb);
+        const uint64_t later = (lane < 63 ? (heads & (~0ull << (lane + 1))) : 0ull) & segm;
+        const int e = later ? (int)__builtin_ctzll(later) - sb : n;
+        group_mask<NW>(order + sb, meta, msk, p, e, w_at, rule, res);
+    }
+    const int prev_x = 2 * prev_cx - 2;
+    const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
+    const int cnt = (merged ? 1 : 0) + (span ? 1 : 0);
+    const int ic = dpp_scan_seg<S>(cnt, 0, OpAdd());
+    const int idx = ic - cnt;
+    const size_t ib = 2 * (size_t)base;
+    if (span) {
+        const int x0 = prev_x + 2;
+        A.items[ib + idx] = (uint32_t)x0 | ((uint32_t)(x - x0) << 16);
+    }
+    if (merged) {
+        A.items[ib + idx + (span ? 1 : 0)] = (uint32_t)x;
+        if (A.imask) store_mask<NW>(A.imask, ib + idx + (span ? 1 : 0), res);
+        A.icov[ib + idx + (span ? 1 : 0)] = cell_counts<SPP>(res);
+    }
+    const int lo = span ? prev_x + 2 : (merged ? x : INT_MAX);
+    const int hi = merged ? x + 2 : (span ? x : INT_MIN);
+    const int xmin = dpp_scan_seg<S>(lo, INT_MAX, OpMin());
+    const int xmax = dpp_scan_seg<S>(hi, INT_MIN, OpMax());
+    const int nmerged = dpp_scan_seg<S>(merged ? 1 : 0, 0, OpAdd());
+    const uint64_t tb = tile_bits(lo, hi, W);
+    const uint32_t tlo = (uint32_t)dpp_scan_seg<S>((int)(uint32_t)tb, 0, OpOr());
+    const uint32_t thi = (uint32_t)dpp_scan_seg<S>((int)(uint32_t)(tb >> 32), 0, OpOr());
+    if (k >= 0 && i == S - 1) {
+        // the segment's last lane holds its bucket's totals
+        const int64_t b = b0 + (int64_t)k * G;
+        A.bicnt[b] = (uint32_t)ic;
+        A.bmcnt[b] = (uint32_t)nmerged;
+        A.bext[b] = ic ? ((uint32_t)xmin | ((uint32_t)xmax << 16)) : 0xFFFFu;
+        A.btile[b] = (uint64_t)tlo | ((uint64_t)thi << 32);
+    }
+    __syncthreads();
+}
+
 // Every bucket: empty ones are cleared, buckets of at most 128 fragments are resolved,
 // larger ones are listed for the size-class passes below.
 template <int SPP>
@@ -513,16 +620,47 @@ __global__ void __launch_bounds__(64, 6) k_resolve_first(ResolveArgs A) {
         }
         const int64_t left = (A.NB - b0 + G - 1) / G;
         const int nk = (int)(left < 64 ? left : 64);
+        // buckets of up to 16 fragments four to a wave, of 17..32 two to a wave
+        const int myn = lane < nk ? vend - vbase : -1;
+        uint64_t m16 = __ballot(myn > 0 && myn <= 16);
+        uint64_t m32 = __ballot(myn > 16 && myn <= 32);
+        uint64_t rest = __ballot(lane < nk) & ~(m16 | m32);
+        while (m16) {
+            int ks[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                ks[s] = m16 ? (int)__builtin_ctzll(m16) : -1;
+                m16 &= m16 - 1ull;
+            }
+            resolve_pack<SPP, 16>(A, b0, G, ks, vbase, vend, vcarry, vrule, vW, order, meta, msk);
+        }
+        while (m32) {
+            int ks[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                ks[s] = m32 ? (int)__builtin_ctzll(m32) : -1;
+                m32 &= m32 - 1ull;
+            }
+            resolve_pack<SPP, 32>(A, b0, G, ks, vbase, vend, vcarry, vrule, vW, order, meta, msk);
+        }
+        // the rest one by one, the next bucket's records loading during the current one
         SmallIn<NW> in0, in1;
-        small_load<NW>(A, (uint32_t)__builtin_amdgcn_readlane(vbase, 0), (uint32_t)__builtin_amdgcn_readlane(vend, 0),
-                       in0);
-        for (int k = 0; k < nk; ++k) {
+        if (rest) {
+            const int k0 = (int)__builtin_ctzll(rest);
+            small_load<NW>(A, (uint32_t)__builtin_amdgcn_readlane(vbase, k0),
+                           (uint32_t)__builtin_amdgcn_readlane(vend, k0), in0);
+        }
+        while (rest) {
+            const int k = (int)__builtin_ctzll(rest);
+            rest &= rest - 1ull;
             const int64_t b = b0 + (int64_t)k * G;
             const uint32_t base = (uint32_t)__builtin_amdgcn_readlane(vbase, k);
             const int n = __builtin_amdgcn_readlane(vend, k) - (int)base;
-            if (k + 1 < nk)
-                small_load<NW>(A, (uint32_t)__builtin_amdgcn_readlane(vbase, k + 1),
-                               (uint32_t)__builtin_amdgcn_readlane(vend, k + 1), in1);
+            if (rest) {
+                const int kn = (int)__builtin_ctzll(rest);
+                small_load<NW>(A, (uint32_t)__builtin_amdgcn_readlane(vbase, kn),
+                               (uint32_t)__builtin_amdgcn_readlane(vend, kn), in1);
+            }
             if (n == 0) {
                 if (lane == 0) { A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; A.btile[b] = 0ull; }
             } else if (n > 128) {
