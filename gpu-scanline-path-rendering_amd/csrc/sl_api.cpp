@@ -72,7 +72,7 @@ struct sl_handle {
     // per-frame arena
     DevBuf b_pcnt, b_nchain, b_choff, b_chains, b_chain_t;
     DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
-        b_boff, b_bwsum, b_bcarry, b_bcursor, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
+        b_boff, b_bwsum, b_bcarry, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
         b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
         b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_image, b_scan_tmp, b_lcount;
     // debug host copies
@@ -356,7 +356,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         std::max<size_t>({(size_t)C + 1, (size_t)P + 1, (size_t)NB + 1, 256 * rs_tiles + 1, (size_t)F + 1}));
     if ((rc = ensure(h, h->b_binfo, (size_t)NB * 8)) || (rc = ensure(h, h->b_bcount, ((size_t)NB + 1) * 4)) ||
         (rc = ensure(h, h->b_boff, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bwsum, ((size_t)NB + 1) * 4)) ||
-        (rc = ensure(h, h->b_bcarry, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bcursor, (size_t)NB * 4)) ||
+        (rc = ensure(h, h->b_bcarry, ((size_t)NB + 1) * 4)) ||
         (rc = ensure(h, h->b_rec_meta, (size_t)F * 4)) || (rc = ensure(h, h->b_rec_mask, (size_t)F * 4 * NW)) ||
         (rc = ensure(h, h->b_scratch, (size_t)F * 8)) || (rc = ensure(h, h->b_items, (size_t)F * 8)) ||
         (rc = ensure(h, h->b_imask, (size_t)F * 8 * NW)) || (rc = ensure(h, h->b_icov, (size_t)F * 8)) || (rc = ensure(h, h->b_bicnt, ((size_t)NB + 1) * 4)) ||
@@ -371,7 +371,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
         return rc;
     f.binfo = P_<int2>(h->b_binfo); f.bcount = P_<uint32_t>(h->b_bcount); f.boff = P_<uint32_t>(h->b_boff);
-    f.bwsum = P_<uint32_t>(h->b_bwsum); f.bcarry = P_<uint32_t>(h->b_bcarry); f.bcursor = P_<uint32_t>(h->b_bcursor);
+    f.bwsum = P_<uint32_t>(h->b_bwsum); f.bcarry = P_<uint32_t>(h->b_bcarry);
     f.rec_meta = P_<uint32_t>(h->b_rec_meta); f.rec_mask = P_<uint32_t>(h->b_rec_mask);
     f.scratch_keys = P_<uint32_t>(h->b_scratch); f.items = P_<uint32_t>(h->b_items); f.imask = P_<uint32_t>(h->b_imask);
     f.icov = P_<uint32_t>(h->b_icov);
@@ -414,8 +414,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     // element counts (M, S) and the error flag
     unsigned long long *red = (unsigned long long *)(P_<uint32_t>(h->b_small) + 4);
     sl::launch_fill_u32((uint32_t *)red, 0u, 4, s);
-    sl::launch_count_reduce(f.bmcnt, NB, red, s);
-    sl::launch_count_reduce(f.bicnt, NB, red + 1, s);
+    sl::launch_count_reduce(f.bmcnt, f.bicnt, NB, red, s);
     uint32_t small[8];
     HIPCHK(h, hipMemcpyAsync(small, h->b_small.p, 32, hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
@@ -532,7 +531,7 @@ int sl_destroy(sl_handle *h) {
     DevBuf *bufs[] = {&h->b_pcnt, &h->b_nchain, &h->b_choff, &h->b_chains, &h->b_chain_t, &h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->frule, &h->finfo,
                       &h->ptable[0], &h->ptable[1], &h->ptable[2], &h->ctab[0], &h->ctab[1], &h->b_tvtx, &h->b_pflags,
                       &h->b_prow, &h->b_ccuts, &h->b_cncut, &h->b_cfrag, &h->b_coff, &h->b_ts, &h->b_pnb, &h->b_pbase,
-                      &h->b_binfo, &h->b_bcount, &h->b_boff, &h->b_bwsum, &h->b_bcarry, &h->b_bcursor, &h->b_rec_meta,
+                      &h->b_binfo, &h->b_bcount, &h->b_boff, &h->b_bwsum, &h->b_bcarry, &h->b_rec_meta,
                       &h->b_rec_mask, &h->b_scratch, &h->b_items, &h->b_imask, &h->b_icov, &h->b_bicnt, &h->b_bmcnt, &h->b_bext,
                       &h->b_big_list, &h->b_small, &h->b_nz_flag, &h->b_nz_off, &h->b_nzk0, &h->b_nzk1, &h->b_nzv0,
                       &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp,

@@ -279,21 +279,23 @@ __global__ void k_path_buckets(uint32_t P, const int2 *__restrict__ prow, uint32
     pnb[p] = (uint32_t)(nhi + 1 + (pr.x == 0 ? 1 : 0));
 }
 
-// per bucket: owning path and row; clears the per-bucket counters
-__global__ void k_bucket_info(uint32_t P, const int2 *__restrict__ prow, const uint32_t *__restrict__ pbase,
-                              int2 *binfo, uint32_t *bcount, uint32_t *bwsum, uint32_t *bcursor) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+// per bucket: owning path and row; clears the per-bucket counters.  One wavefront per path,
+// its lanes over the path's buckets (coalesced).
+__global__ void __launch_bounds__(256) k_bucket_info(uint32_t P, const int2 *__restrict__ prow,
+                                                     const uint32_t *__restrict__ pbase, int2 *binfo, uint32_t *bcount,
+                                                     uint32_t *bwsum) {
+    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (p >= P) return;
     const int2 pr = prow[p];
     const int lo1 = tmax(pr.x, 1);
     const int nhi = (pr.y >= lo1) ? pr.y - lo1 + 1 : 0;
     const uint32_t b0 = pbase[p], nb = pbase[p + 1] - b0;
-    for (uint32_t j = 0; j < nb; ++j) {
-        int row = (int)j < nhi ? lo1 + (int)j : ((int)j == nhi ? -1 : 0);
+    for (uint32_t j = lane; j < nb; j += 64) {
+        const int row = (int)j < nhi ? lo1 + (int)j : ((int)j == nhi ? -1 : 0);
         binfo[b0 + j] = make_int2((int)p, row);
         bcount[b0 + j] = 0;
         bwsum[b0 + j] = 0;
-        bcursor[b0 + j] = 0;
     }
 }
 
@@ -540,8 +542,8 @@ void launch_path_buckets(const SceneDev &sc, Frame &f, hipStream_t s) {
 
 void launch_bucket_info(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.P) return;
-    hipLaunchKernelGGL(k_bucket_info, dim3(blocks_for(sc.P, 64)), dim3(64), 0, s, sc.P, (const int2 *)f.prow,
-                       (const uint32_t *)f.pbase, f.binfo, f.bcount, f.bwsum, f.bcursor);
+    hipLaunchKernelGGL(k_bucket_info, dim3(blocks_for(sc.P, 4)), dim3(256), 0, s, sc.P, (const int2 *)f.prow,
+                       (const uint32_t *)f.pbase, f.binfo, f.bcount, f.bwsum);
 }
 
 void launch_frag_count(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {

@@ -87,7 +87,6 @@ struct Frame {
     uint32_t *boff;          // [NB+1]
     uint32_t *bwsum;         // [NB+1] per-bucket sum of scan windings -> bcarry
     uint32_t *bcarry;        // [NB+1]
-    uint32_t *bcursor;       // [NB]
     uint32_t *frank;         // [F] rank of each fragment in its bucket (~0: none)
     uint32_t *rec_meta;      // [F] (valid fragments only are written)
     uint32_t *rec_mask;      // [F * words]
@@ -163,6 +162,7 @@ void launch_debug_fragments(const SceneDev &sc, const Tables &t, Frame &f, int32
                             int8_t *wn, uint32_t *mask4, hipStream_t s);
 void launch_debug_draw_list(const SceneDev &sc, const Tables &t, Frame &f, const uint32_t *moff,
                             const uint32_t *ioff, int4 *draw, uint32_t *mmask4, hipStream_t s);
-void launch_count_reduce(const uint32_t *a, size_t n, unsigned long long *out, hipStream_t s);
+// out[0] += sum of a, out[1] += sum of c
+void launch_count_reduce(const uint32_t *a, const uint32_t *c, size_t n, unsigned long long *out, hipStream_t s);
 
 }  // namespace sl

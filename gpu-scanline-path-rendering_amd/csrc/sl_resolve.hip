@@ -1195,17 +1195,18 @@ __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
 // decoded floats of its 8-bit codes; every blend re-quantises to a code and decodes it
 // again, exactly as an 8-bit framebuffer does:
 //   UNORM8: code = largest k with v >= (k - 0.5) / 255, i.e. with 510 v - (2k - 1) >= 0 --
-//           one fma gives that sign exactly; decode = RN(k / 255) by a corrected product.
+//           one fma gives that sign exactly, applied to an estimate that is never too
+//           small; decode = RN(k / 255) by a corrected product.
 //   sRGB8:  a 4096-entry table gives the code k0 of the interval start floor(4096 v)/4096;
 //           no interval holds two thresholds, so the code is k0 or k0 + 1, and one 16-byte
 //           step entry {thr[k0+1], dec[k0], dec[k0+1]} decides and decodes.
 // ---------------------------------------------------------------------------------
+// The code is m = floor(255 v + 0.5) (clamped).  The estimate trunc(RN(255 v + 0.5)) is m or
+// m + 1: rounding is monotone and the integers m, m + 1 are floats, so RN never crosses below
+// m.  One exact sign test, RN(510 v + 1 - 2k) < 0 iff v < thr[k], takes the extra one back.
 __device__ __forceinline__ int unorm_code(float v) {
     int k = tmin(tmax((int)__builtin_fmaf(v, 255.f, 0.5f), 0), 255);
-    const bool below = __builtin_fmaf(v, 510.f, (float)(1 - 2 * k)) < 0.f;        // v < thr[k]
-    const bool above = __builtin_fmaf(v, 510.f, (float)(-1 - 2 * k)) >= 0.f;      // v >= thr[k+1]
-    k += (above && k < 255) ? 1 : 0;
-    k -= below ? 1 : 0;
+    k -= (__builtin_fmaf(v, 510.f, (float)(1 - 2 * k)) < 0.f) ? 1 : 0;  // v < thr[k]
     return tmax(k, 0);
 }
 
@@ -1548,12 +1549,28 @@ __global__ void k_debug_draw(int64_t NB, const int2 *__restrict__ binfo, const u
     }
 }
 
-__global__ void k_count_reduce(const uint32_t *__restrict__ a, size_t n, unsigned long long *out) {
-    unsigned long long s = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) s += a[i];
+__global__ void __launch_bounds__(256) k_count_reduce(const uint32_t *__restrict__ a, const uint32_t *__restrict__ c,
+                                                      size_t n, unsigned long long *out) {
+    __shared__ unsigned long long part[2][4];
+    unsigned long long sa = 0, sc = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        sa += a[i];
+        sc += c[i];
+    }
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+    for (int d = 32; d >= 1; d >>= 1) {
+        sa += __shfl_xor(sa, d, 64);
+        sc += __shfl_xor(sc, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = sa;
+        part[1][threadIdx.x >> 6] = sc;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const unsigned long long t = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] + part[threadIdx.x][3];
+        atomicAdd(&out[threadIdx.x], t);
+    }
 }
 
 inline unsigned blocks_for(int64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -1715,8 +1732,8 @@ void launch_debug_draw_list(const SceneDev &sc, const Tables &t, Frame &f, const
                            (const uint32_t *)f.bicnt, (const uint32_t *)f.items, (const uint32_t *)f.imask, moff, ioff, draw, mmask4);
 }
 
-void launch_count_reduce(const uint32_t *a, size_t n, unsigned long long *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_count_reduce, dim3(256), dim3(256), 0, s, a, n, out);
+void launch_count_reduce(const uint32_t *a, const uint32_t *c, size_t n, unsigned long long *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_reduce, dim3(256), dim3(256), 0, s, a, c, n, out);
 }
 
 }  // namespace sl
