@@ -1383,11 +1383,17 @@ __global__ void __launch_bounds__(256, 8) k_composite_cov(CovArgs A) {
             for (int g = 0; g < kCovGroup; ++g) sv[g] = slot[w][g][lane];
 #pragma unroll
             for (int g = 0; g < kCovGroup; ++g) sv[g] = (uint32_t)dpp_scan((int)sv[g], 0, OpMax());
+            // Each lane lists the group's buckets covering its column, in path order, as
+            // {n0, n1, g}; the blend passes then run as many times as the deepest lane's
+            // list, each lane blending its own next entry (same per-pixel order as one
+            // pass per bucket, fewer passes where the buckets do not overlap).
+            uint32_t ent[kCovGroup];
+            int nent = 0;
 #pragma unroll
             for (int g = 0; g < kCovGroup; ++g) {
+                ent[g] = 0u;
                 if (g >= ng) break;
                 const uint2 ref = l_ref[w][g0 + g];
-                const float4 src = l_src[w][g0 + g];
                 const int p = (int)sv[g] - 1;
                 uint32_t pit, pcv;
                 if (ref.y > 64u) {
@@ -1399,14 +1405,25 @@ __global__ void __launch_bounds__(256, 8) k_composite_cov(CovArgs A) {
                 }
                 const int px0 = (int)(pit & 0xFFFFu), pw = (int)(pit >> 16);
                 const int pend = px0 + (pw ? pw : 2);
-                if (p >= 0 && x < pend && x < W) {
-                    uint32_t n0 = SPP, n1 = SPP;
-                    if (pw == 0) {
-                        n0 = (pcv >> (6 * ((x - px0) * 2))) & 63u;
-                        n1 = (pcv >> (6 * ((x - px0) * 2 + 1))) & 63u;
-                    }
-                    blend_dec<SRGB>(s_step, s_lut, d0, src, src.w * ((float)n0 / (float)SPP));
-                    blend_dec<SRGB>(s_step, s_lut, d1, src, src.w * ((float)n1 / (float)SPP));
+                const bool covered = p >= 0 && x < pend && x < W;
+                uint32_t n0 = SPP, n1 = SPP;
+                if (pw == 0) {
+                    n0 = (pcv >> (6 * ((x - px0) * 2))) & 63u;
+                    n1 = (pcv >> (6 * ((x - px0) * 2 + 1))) & 63u;
+                }
+                const uint32_t e = n0 | (n1 << 8) | ((uint32_t)g << 16);
+#pragma unroll
+                for (int j = 0; j <= g; ++j) ent[j] = (covered && nent == j) ? e : ent[j];
+                nent += covered ? 1 : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kCovGroup; ++j) {
+                if (!__ballot(nent > j)) break;
+                if (nent > j) {
+                    const uint32_t e = ent[j];
+                    const float4 src = l_src[w][g0 + (int)(e >> 16)];
+                    blend_dec<SRGB>(s_step, s_lut, d0, src, src.w * ((float)(e & 63u) / (float)SPP));
+                    blend_dec<SRGB>(s_step, s_lut, d1, src, src.w * ((float)((e >> 8) & 63u) / (float)SPP));
                 }
             }
             wave_sync_lds();
