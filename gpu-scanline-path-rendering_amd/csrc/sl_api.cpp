@@ -65,7 +65,7 @@ struct sl_handle {
     uint32_t last_F = 0, last_NCH = 0, last_NB = 0;
 
     // scene (device)
-    DevBuf vtx, vpath, cvpos, ctype, cpath, cw, crev, frule, finfo;
+    DevBuf vtx, vpath, cvpos, ctype, cpath, cw, crev, cinfo, frule, finfo;
     // tables (device)
     DevBuf ptable[3];
     DevBuf ctab[2];  // [srgb] : dec_c, thr_c, dec_a, thr_a, lut_c, lut_a packed
@@ -217,7 +217,7 @@ sl::SceneDev scene_view(sl_handle *h) {
     s.P = h->P; s.C = h->C; s.V = h->V;
     s.vtx = P_<float2>(h->vtx); s.vpath = P_<uint32_t>(h->vpath);
     s.cvpos = P_<uint32_t>(h->cvpos); s.ctype = P_<uint8_t>(h->ctype); s.cpath = P_<uint32_t>(h->cpath);
-    s.cw = P_<float>(h->cw); s.crev = P_<uint8_t>(h->crev);
+    s.cw = P_<float>(h->cw); s.crev = P_<uint8_t>(h->crev); s.cinfo = P_<uint4>(h->cinfo);
     s.frule = P_<uint8_t>(h->frule); s.finfo = P_<uint32_t>(h->finfo);
     s.pcanvas = h->nscenes ? P_<int4>(h->pcanvas) : nullptr;
     return s;
@@ -528,7 +528,7 @@ int sl_destroy(sl_handle *h) {
     if (!h) return SL_OK;
     (void)hipSetDevice(h->device);
     (void)hipDeviceSynchronize();
-    DevBuf *bufs[] = {&h->b_pcnt, &h->b_nchain, &h->b_choff, &h->b_chains, &h->b_chain_t, &h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->frule, &h->finfo,
+    DevBuf *bufs[] = {&h->b_pcnt, &h->b_nchain, &h->b_choff, &h->b_chains, &h->b_chain_t, &h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->cinfo, &h->frule, &h->finfo,
                       &h->ptable[0], &h->ptable[1], &h->ptable[2], &h->ctab[0], &h->ctab[1], &h->b_tvtx, &h->b_pflags,
                       &h->b_prow, &h->b_ccuts, &h->b_cncut, &h->b_cfrag, &h->b_coff, &h->b_ts, &h->b_pnb, &h->b_pbase,
                       &h->b_binfo, &h->b_bcount, &h->b_boff, &h->b_bwsum, &h->b_bcarry, &h->b_rec_meta,
@@ -623,6 +623,15 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
     }
     for (uint32_t i = 0; i < V; ++i)
         if (vpath[i] == 0xFFFFFFFFu) vpath[i] = 0;  // vertices of no curve: harmless, never read
+    std::vector<uint32_t> cinfo((size_t)C * 4);
+    for (uint32_t c = 0; c < C; ++c) {
+        uint32_t wb;
+        std::memcpy(&wb, &cw[c], 4);
+        cinfo[4 * (size_t)c] = s->curve_vertex_pos[c];
+        cinfo[4 * (size_t)c + 1] = s->curve_path_id[c];
+        cinfo[4 * (size_t)c + 2] = (uint32_t)s->curve_type[c] | ((uint32_t)crev[c] << 8);
+        cinfo[4 * (size_t)c + 3] = wb;
+    }
     std::vector<uint32_t> finfo(P);
     for (uint32_t p = 0; p < P; ++p) {
         const uint32_t c = s->fill_color[p];
@@ -633,7 +642,8 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
     if ((rc = upload(h, h->vtx, vtx.data(), vtx.size() * 4)) || (rc = upload(h, h->vpath, vpath.data(), (size_t)V * 4)) ||
         (rc = upload(h, h->cvpos, s->curve_vertex_pos, (size_t)C * 4)) || (rc = upload(h, h->ctype, s->curve_type, C)) ||
         (rc = upload(h, h->cpath, s->curve_path_id, (size_t)C * 4)) || (rc = upload(h, h->cw, cw.data(), (size_t)C * 4)) ||
-        (rc = upload(h, h->crev, crev.data(), C)) || (rc = upload(h, h->frule, s->fill_rule, P)) ||
+        (rc = upload(h, h->crev, crev.data(), C)) || (rc = upload(h, h->cinfo, cinfo.data(), (size_t)C * 16)) ||
+        (rc = upload(h, h->frule, s->fill_rule, P)) ||
         (rc = upload(h, h->finfo, finfo.data(), (size_t)P * 4)))
         return rc;
     h->P = P; h->C = C; h->V = (P == 0) ? 0 : V;

@@ -116,13 +116,12 @@ __device__ __forceinline__ void frag_load_ts(const int2 *__restrict__ ts, int64_
 }
 
 __device__ __forceinline__ void frag_load_curve(const SceneDev &sc, FragIn &in) {
-    const uint32_t c = (uint32_t)in.e0.x;
-    in.path = sc.cpath[c];
-    in.type = sc.ctype[c];
-    const float cw = sc.cw[c];
-    in.w = (in.type == CT_RATIONAL) ? cw : 0.f;
-    in.rev = sc.crev[c] != 0;
-    in.vp = sc.cvpos[c];
+    const uint4 ci = sc.cinfo[(uint32_t)in.e0.x];
+    in.vp = ci.x;
+    in.path = ci.y;
+    in.type = (int)(ci.z & 0xFFu);
+    in.rev = (ci.z >> 8) != 0u;
+    in.w = (in.type == CT_RATIONAL) ? bf(ci.w) : 0.f;
 }
 
 __device__ __forceinline__ void frag_load_verts(const float2 *__restrict__ tv, FragIn &in) {

@@ -185,9 +185,10 @@ struct Curve {
 
 __device__ __forceinline__ Curve load_curve(const SceneDev &sc, const float2 *__restrict__ tv, uint32_t c) {
     Curve cu;
-    cu.type = sc.ctype[c];
-    cu.w = (cu.type == CT_RATIONAL) ? sc.cw[c] : 0.f;
-    const uint32_t p = sc.cvpos[c];
+    const uint4 ci = sc.cinfo[c];
+    cu.type = (int)(ci.z & 0xFFu);
+    cu.w = (cu.type == CT_RATIONAL) ? bf(ci.w) : 0.f;
+    const uint32_t p = ci.x;
     const int nv = cu.type & 7;
     float2 a = tv[p], b = tv[p + 1];
     float2 d = nv > 2 ? tv[p + 2] : make_float2(0.f, 0.f);

@@ -27,6 +27,8 @@ struct SceneDev {
     const uint32_t *cpath;   // [C]
     const float *cw;         // [C] rational weight (else -1)
     const uint8_t *crev;     // [C] 1 = vertex order was reversed at upload
+    const uint4 *cinfo;      // [C] the same per curve in one record: {vertex pos, path,
+                             //     type | reversed << 8, rational weight bits}
     const uint8_t *frule;    // [P]
     const uint32_t *finfo;   // [P] fill_info (RGBA8, alpha 0 -> 0)
     const int4 *pcanvas;     // [P] batched scenes: {W, H, first cell row, scene} of the
