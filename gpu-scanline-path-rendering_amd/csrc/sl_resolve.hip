@@ -1297,7 +1297,7 @@ __device__ __noinline__ void scatter_big(const uint32_t *items, uint32_t ib, int
 // coverage alpha = paint alpha * covered samples / N (output_{8,32}.frag.glsl).
 // The kernel is bound by VALU issue, LDS and load latency together; eight waves per SIMD
 // (registers capped at 64, a few spilled) hide the latency best.
-template <int SPP, bool SRGB, int kCovGroup>
+template <int SPP, bool SRGB, int kCovGroup, bool BATCH>
 __global__ void __launch_bounds__(256, 8) k_composite_cov(CovArgs A) {
     __shared__ float4 s_step[SRGB ? 256 : 1];
     __shared__ uint8_t s_lut[SRGB ? 4096 : 4];
@@ -1314,7 +1314,7 @@ __global__ void __launch_bounds__(256, 8) k_composite_cov(CovArgs A) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int W = A.W, H = A.H, tiles_x = A.tiles_x, row0 = 0, pitch = A.pitch;
     int tile;
-    if (A.nscenes == 0) {
+    if (!BATCH) {
         // Workgroup b runs on XCD b % 8 (round-robin deal): XCD x takes the cell rows
         // x, x + 8, x + 16, ... whole, its workgroups walking them tile by tile, so each
         // row's items are fetched into one XCD's L2 and every XCD gets rows from the whole
@@ -1328,7 +1328,7 @@ __global__ void __launch_bounds__(256, 8) k_composite_cov(CovArgs A) {
         if (tile >= A.ntiles) return;
     }
     uint8_t *image = A.image;
-    if (A.nscenes) {
+    if (BATCH) {
         // the scene whose tile range holds this tile: last one with first tile <= tile
         int lo = 0, hi = A.nscenes - 1;
         while (lo < hi) {
@@ -1684,8 +1684,13 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
         dim3 g(f.nscenes ? blocks_for(ntiles, 4) : 8u * blocks_for((int64_t)tiles_x * ((R + 7) / 8), 4)), b(256);
 #define SL_LAUNCH(N, SR)                                                                   \
     do {                                                                                   \
-        if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4>), g, b, 0, s, C); \
-        else hipLaunchKernelGGL((k_composite_cov<N, SR, 8>), g, b, 0, s, C);               \
+        if (f.nscenes) {                                                                   \
+            if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4, true>), g, b, 0, s, C); \
+            else hipLaunchKernelGGL((k_composite_cov<N, SR, 8, true>), g, b, 0, s, C);               \
+        } else {                                                                           \
+            if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4, false>), g, b, 0, s, C); \
+            else hipLaunchKernelGGL((k_composite_cov<N, SR, 8, false>), g, b, 0, s, C);               \
+        }                                                                                  \
     } while (0)
         if (f.srgb) {
             if (t.spp == 8) SL_LAUNCH(8, true);
