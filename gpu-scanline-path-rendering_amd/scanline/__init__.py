@@ -4,9 +4,10 @@ from ._abi import (CURVE_CUBIC, CURVE_LINEAR, CURVE_QUADRATIC, CURVE_RATIONAL, F
 from .batch import BatchRasterizer, batch_workload
 from .rasterizer import VGRasterizer, createRasterizer
 from .scene import VGContainer, SceneArrays, config_scene, generate, rgba
+from . import rvg
 
 __all__ = [
     "CURVE_CUBIC", "CURVE_LINEAR", "CURVE_QUADRATIC", "CURVE_RATIONAL", "FILL_EVEN_ODD",
     "FILL_NONZERO", "ScanlineError", "lib", "VGRasterizer", "createRasterizer", "VGContainer",
-    "SceneArrays", "config_scene", "generate", "rgba", "BatchRasterizer", "batch_workload",
+    "SceneArrays", "config_scene", "generate", "rgba", "BatchRasterizer", "batch_workload", "rvg",
 ]

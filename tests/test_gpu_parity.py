@@ -4,6 +4,8 @@ Bar (per stage): intersections, raw fragment keys / path ids / winding / masks, 
 per-sample masks and the draw list are bit-exact; the final RGBA8 image is within 1/255
 per channel (it is in fact expected to be identical, as both sides quantise identically).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -182,3 +184,14 @@ def test_text_like_scene_full_image(rasterizer):
     for k, v in ref.counts().items():
         assert cnt[k] == v, (k, cnt[k], v)
     np.testing.assert_array_equal(rasterizer.output(), ref.get("image").reshape(H, W, 4))
+
+
+@pytest.mark.parametrize("spp", [8, 32])
+def test_rvg_sample_scene(rasterizer, spp):
+    """A scene read by the RVG loader (tests/golden/sample.rvg: lines, quadratic, cubic,
+    half-circle and rational arcs, an affine transform, both fill rules)."""
+    from scanline import rvg
+    vg, _, _ = rvg.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sample.rvg"))
+    cnt, exact = compare(rasterizer, vg, 64, 48, spp, label="rvg sample")
+    assert cnt["curve_fragment"] > 0
+    assert exact == 1.0
