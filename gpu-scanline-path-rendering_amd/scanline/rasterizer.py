@@ -97,6 +97,12 @@ class VGRasterizer:
         """stream: None or an integer hipStream_t (e.g. torch.cuda.current_stream().cuda_stream)."""
         self._chk(self._L.sl_rasterize(self._h, ctypes.c_void_p(stream or 0)))
 
+    def saveOutputToFile(self, fileName):
+        """saveOutputToFile (ras_base.cpp:1221-1277): the image with rows flipped and alpha
+        forced to 255, as a PNG file."""
+        from . import png
+        png.write(fileName, self.output(flip=True))
+
     def output(self, flip=False):
         """The RGBA8 image as an (H, W, 4) uint8 array; flip=True gives saveOutputToFile's
         top-down rows with alpha forced to 255 (ras_base.cpp:1249-1262)."""

@@ -195,3 +195,18 @@ def test_rvg_sample_scene(rasterizer, spp):
     cnt, exact = compare(rasterizer, vg, 64, 48, spp, label="rvg sample")
     assert cnt["curve_fragment"] > 0
     assert exact == 1.0
+
+
+def test_save_output_to_file(rasterizer, tmp_path):
+    """saveOutputToFile: the PNG holds the image with rows flipped and alpha 255."""
+    from scanline import png
+    vg = VGContainer()
+    vg.addPath(0, rgba(20, 200, 90, 128))
+    vg.addPolygon([(3, 2), (40, 5), (30, 28)])
+    render_gpu(rasterizer, vg.arrays(), 48, 32, 8, True, False, None)
+    path = str(tmp_path / "out.png")
+    rasterizer.saveOutputToFile(path)
+    img = png.decode(open(path, "rb").read())
+    np.testing.assert_array_equal(img, rasterizer.output(flip=True))
+    assert (img[..., 3] == 255).all()
+    np.testing.assert_array_equal(img[::-1, :, :3], rasterizer.output()[..., :3])
