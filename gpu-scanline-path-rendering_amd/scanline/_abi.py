@@ -65,6 +65,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError("libscanline.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    # torch wheels bundle their own HIP runtime under the same soname (libamdhip64.so.7).
+    # Loaded first, it also serves this library, so torch's streams and allocations and
+    # the library's calls meet in one runtime; loaded after, it would be a second copy.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i, u64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
     sig = {

@@ -19,7 +19,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _hip():
-    return ctypes.CDLL("libamdhip64.so")
+    # the HIP runtime libscanline.so is bound to (matched by soname once it is loaded)
+    scanline._abi.lib()
+    return ctypes.CDLL("libamdhip64.so.7")
 
 
 @pytest.fixture(scope="module")
