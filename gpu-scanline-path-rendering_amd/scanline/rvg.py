@@ -28,7 +28,7 @@ chooses between raising and skipping those paths.
 
 import numpy as np
 
-from .scene import VGContainer, rgba
+from .scene import FT_COLOR, FT_GRADIENT, FT_NONE, VGContainer, rgba
 
 _F32 = np.float32
 
@@ -133,7 +133,7 @@ def loads(text, a128=False, gradients="raise"):
         if nxt() != "element":
             raise RVGError("expected 'element'")
         if last_path_vertices != len(verts):
-            path = vg.addPath(0, rgba(0, 0, 0))
+            path = vg.addPath(0, rgba(0, 0, 0), FT_NONE)
             last_path_vertices = len(verts)
         rule = nxt()
         if rule == "nzfill":
@@ -233,10 +233,11 @@ def loads(text, a128=False, gradients="raise"):
                 ca = int(np.trunc(_F32(ca) / _F32(255) * _F32(127))) & 0xFF
                 ca = int(np.trunc(_F32(ca) / _F32(127) * _F32(255))) & 0xFF
             vg._color[path] = rgba(cr, cg, cb, ca)
+            vg._ftype[path] = FT_COLOR
         elif kind in ("linear", "radial"):
             if gradients == "raise":
                 raise RVGError("gradient paints are not supported")
-            vg._color[path] = 0  # fill info 0: the path draws nothing
+            vg._ftype[path] = FT_GRADIENT  # not rendered here: the path draws nothing
             skip_line()
         else:
             raise RVGError("unknown paint %r" % kind)

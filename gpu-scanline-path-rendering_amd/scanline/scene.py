@@ -57,6 +57,10 @@ class SceneArrays:
                 self.fill_color)
 
 
+# fill types of a path (VGFillType, vg_container.h:19-23)
+FT_NONE, FT_COLOR, FT_GRADIENT = 0, 1, 4
+
+
 def rgba(r, g, b, a=255):
     """u8rgba packed little-endian (gpu-scanline/src/mochimazui/color.h:29-57)."""
     return (int(r) & 255) | ((int(g) & 255) << 8) | ((int(b) & 255) << 16) | ((int(a) & 255) << 24)
@@ -65,10 +69,12 @@ def rgba(r, g, b, a=255):
 class VGContainer:
     """Host-side scene builder with the reference's VGContainer method names."""
 
-    def __init__(self, width=0, height=0):
-        self.width, self.height = width, height
+    def __init__(self, width=0, height=0, x=0, y=0):
+        self.x, self.y, self.width, self.height = x, y, width, height
         self._v, self._cpos, self._ctype, self._cpath, self._cw = [], [], [], [], []
-        self._rule, self._color = [], []
+        self._rule, self._color, self._ftype, self._findex = [], [], [], []
+        self._cfirst, self._cowner = [], []  # contours: first curve, owning path
+        self.gradient_ramp, self.gradient_table = [], []  # kept for file round trips
 
     def pathNumber(self):
         return len(self._rule)
@@ -79,19 +85,33 @@ class VGContainer:
     def vertexNumber(self):
         return len(self._v)
 
-    def addPath(self, f_rule=0, f_color=rgba(0, 0, 0)):
-        """addPath(f_rule, f_mode, f_color, f_index) (vg_container.h:120) for colour fills."""
+    def contourNumber(self):
+        return len(self._cfirst)
+
+    def addPath(self, f_rule=0, f_color=rgba(0, 0, 0), f_mode=FT_COLOR, f_index=0xFFFFFFFF):
+        """addPath(f_rule, f_mode, f_color, f_index) (vg_container.cpp:310-320); the paint
+        arguments come last here so that colour fills read addPath(rule, colour)."""
         self._rule.append(int(f_rule))
         self._color.append(int(f_color))
+        self._ftype.append(int(f_mode))
+        self._findex.append(int(f_index) & 0xFFFFFFFF)
         return len(self._rule) - 1
 
     def addContour(self):
-        """Contours only group curves in the reference; kept for API parity."""
-        return len(self._ctype)
+        """addContour (vg_container.cpp:269-280): the next curves form a new contour of the
+        last path.  Contours only group curves (the fill does not use them); a curve added
+        to a path with no contour opens one."""
+        if not self._rule:
+            raise ValueError("addPath before addContour")
+        self._cfirst.append(len(self._ctype))
+        self._cowner.append(len(self._rule) - 1)
+        return len(self._cfirst) - 1
 
     def _curve(self, ctype, pts, w=-1.0):
         if not self._rule:
             raise ValueError("addPath before adding curves")
+        if not self._cowner or self._cowner[-1] != len(self._rule) - 1:
+            self.addContour()
         self._cpos.append(len(self._v))
         self._ctype.append(ctype)
         self._cpath.append(len(self._rule) - 1)
@@ -116,10 +136,48 @@ class VGContainer:
         for k in range(len(pts)):
             self.addLinear(pts[k], pts[(k + 1) % len(pts)])
 
+    def fillInfo(self):
+        """Per-path fill word as the rasterizer's input gathers it (ras_base.cpp:144-157):
+        colour fills keep their colour unless its alpha is 0, every other fill type gives 0
+        (gradient paints are not rendered here, so such paths draw nothing)."""
+        return [c if t == FT_COLOR and (c & 0xFF000000) else 0 for c, t in zip(self._color, self._ftype)]
+
     def arrays(self):
         v = np.array(self._v, dtype=np.float32).reshape(-1, 2)
         return SceneArrays(v, self._cpos, self._ctype, self._cpath, self._cw, self._rule,
-                           self._color)
+                           self.fillInfo())
+
+    @classmethod
+    def from_arrays(cls, a, width=0, height=0):
+        """A container holding the scene arrays `a` (e.g. a generated scene), one contour
+        per path, colour fills; curves must be grouped by path in path order."""
+        pid = np.asarray(a.curve_path_id, dtype=np.int64)
+        if len(pid) and (np.diff(pid) < 0).any():
+            raise ValueError("curves are not grouped by path")
+        vg = cls(width, height)
+        vg._v = [tuple(p) for p in np.asarray(a.vertex, dtype=np.float32).astype(np.float64).tolist()]
+        vg._cpos = np.asarray(a.curve_vertex_pos, dtype=np.int64).tolist()
+        vg._ctype = np.asarray(a.curve_type, dtype=np.int64).tolist()
+        vg._cpath = pid.tolist()
+        vg._cw = np.asarray(a.curve_arc_w, dtype=np.float32).astype(np.float64).tolist()
+        vg._rule = np.asarray(a.fill_rule, dtype=np.int64).tolist()
+        vg._color = np.asarray(a.fill_color, dtype=np.int64).tolist()
+        vg._ftype = [FT_COLOR] * len(vg._rule)
+        vg._findex = [0xFFFFFFFF] * len(vg._rule)
+        owners, first = np.unique(pid, return_index=True)
+        vg._cfirst, vg._cowner = first.tolist(), owners.tolist()
+        return vg
+
+    def save(self, filename):
+        """VGContainer::save (vg_container.cpp:1667-1677): `.vgt` text or `.vgb` binary."""
+        from . import vgfile
+        vgfile.save(self, filename)
+
+    @staticmethod
+    def load(filename, gradients="raise"):
+        """VGContainer::load (vg_container.cpp:1679-1689) as a constructor: `.vgt` or `.vgb`."""
+        from . import vgfile
+        return vgfile.load(filename, gradients=gradients)
 
 
 KIND_CUBIC_CHAINS, KIND_MIXED, KIND_GLYPHS = 1, 2, 3

@@ -210,3 +210,20 @@ def test_save_output_to_file(rasterizer, tmp_path):
     np.testing.assert_array_equal(img, rasterizer.output(flip=True))
     assert (img[..., 3] == 255).all()
     np.testing.assert_array_equal(img[::-1, :, :3], rasterizer.output()[..., :3])
+
+
+@pytest.mark.parametrize("suffix", ["vgb", "vgt"])
+def test_scene_file_round_trip(rasterizer, tmp_path, suffix):
+    """A generated scene saved as a .vgb / .vgt file and loaded back renders as the oracle
+    renders the loaded scene; the binary file gives the original image exactly."""
+    a = scanline.generate(scanline.scene.KIND_MIXED, 320, 240, 150, 0x51CE)
+    fn = str(tmp_path / ("scene." + suffix))
+    VGContainer.from_arrays(a, 320, 240).save(fn)
+    vg = VGContainer.load(fn)
+    cnt, exact = compare(rasterizer, vg, 320, 240, 16, label="scene file " + suffix)
+    assert cnt["curve_fragment"] > 0
+    assert exact == 1.0
+    if suffix == "vgb":
+        img = rasterizer.output().copy()
+        render_gpu(rasterizer, a, 320, 240, 16)
+        np.testing.assert_array_equal(rasterizer.output(), img)
