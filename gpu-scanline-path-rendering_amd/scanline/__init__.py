@@ -3,11 +3,13 @@ from ._abi import (CURVE_CUBIC, CURVE_LINEAR, CURVE_QUADRATIC, CURVE_RATIONAL, F
                    FILL_NONZERO, ScanlineError, lib)
 from .batch import BatchRasterizer, batch_workload
 from .rasterizer import VGRasterizer, createRasterizer
-from .scene import FT_COLOR, FT_GRADIENT, FT_NONE, VGContainer, SceneArrays, config_scene, generate, rgba
-from . import rvg, vgfile
+from .scene import (FT_COLOR, FT_GRADIENT, FT_NONE, ST_COLOR, ST_GRADIENT, ST_NONE, VGContainer, SceneArrays,
+                    config_scene, generate, rgba)
+from . import rvg, svg, vgfile
 
 __all__ = [
     "CURVE_CUBIC", "CURVE_LINEAR", "CURVE_QUADRATIC", "CURVE_RATIONAL", "FILL_EVEN_ODD",
     "FILL_NONZERO", "ScanlineError", "lib", "VGRasterizer", "createRasterizer", "VGContainer",
-    "SceneArrays", "config_scene", "generate", "rgba", "BatchRasterizer", "batch_workload", "rvg", "vgfile", "FT_NONE", "FT_COLOR", "FT_GRADIENT",
+    "SceneArrays", "config_scene", "generate", "rgba", "BatchRasterizer", "batch_workload", "rvg", "svg", "vgfile",
+    "FT_NONE", "FT_COLOR", "FT_GRADIENT", "ST_NONE", "ST_COLOR", "ST_GRADIENT",
 ]

@@ -57,8 +57,9 @@ class SceneArrays:
                 self.fill_color)
 
 
-# fill types of a path (VGFillType, vg_container.h:19-23)
+# fill and stroke types of a path (VGFillType / VGStrokeType, vg_container.h:19-29)
 FT_NONE, FT_COLOR, FT_GRADIENT = 0, 1, 4
+ST_NONE, ST_COLOR, ST_GRADIENT = 0, 1, 4
 
 
 def rgba(r, g, b, a=255):
@@ -73,6 +74,7 @@ class VGContainer:
         self.x, self.y, self.width, self.height = x, y, width, height
         self._v, self._cpos, self._ctype, self._cpath, self._cw = [], [], [], [], []
         self._rule, self._color, self._ftype, self._findex = [], [], [], []
+        self._stype, self._scolor = [], []  # stroke type and colour per path
         self._cfirst, self._cowner = [], []  # contours: first curve, owning path
         self.gradient_ramp, self.gradient_table = [], []  # kept for file round trips
 
@@ -95,6 +97,8 @@ class VGContainer:
         self._color.append(int(f_color))
         self._ftype.append(int(f_mode))
         self._findex.append(int(f_index) & 0xFFFFFFFF)
+        self._stype.append(ST_NONE)
+        self._scolor.append(rgba(0, 0, 0))
         return len(self._rule) - 1
 
     def addContour(self):
@@ -137,10 +141,18 @@ class VGContainer:
             self.addLinear(pts[k], pts[(k + 1) % len(pts)])
 
     def fillInfo(self):
-        """Per-path fill word as the rasterizer's input gathers it (ras_base.cpp:144-157):
+        """Per-path fill word as the rasterizer's input gathers it (ras_base.cpp:144-166):
         colour fills keep their colour unless its alpha is 0, every other fill type gives 0
-        (gradient paints are not rendered here, so such paths draw nothing)."""
-        return [c if t == FT_COLOR and (c & 0xFF000000) else 0 for c, t in zip(self._color, self._ftype)]
+        (gradient paints are not rendered here, so such paths draw nothing); a path with a
+        colour stroke is filled with the stroke colour instead, since the reference's
+        renderer takes its input without stroke-to-fill conversion."""
+        out = []
+        for c, t, sc, st in zip(self._color, self._ftype, self._scolor, self._stype):
+            w = c if t == FT_COLOR and (c & 0xFF000000) else 0
+            if st == ST_COLOR:
+                w = sc if sc & 0xFF000000 else 0
+            out.append(w)
+        return out
 
     def arrays(self):
         v = np.array(self._v, dtype=np.float32).reshape(-1, 2)
@@ -164,6 +176,8 @@ class VGContainer:
         vg._color = np.asarray(a.fill_color, dtype=np.int64).tolist()
         vg._ftype = [FT_COLOR] * len(vg._rule)
         vg._findex = [0xFFFFFFFF] * len(vg._rule)
+        vg._stype = [ST_NONE] * len(vg._rule)
+        vg._scolor = [rgba(0, 0, 0)] * len(vg._rule)
         owners, first = np.unique(pid, return_index=True)
         vg._cfirst, vg._cowner = first.tolist(), owners.tolist()
         return vg

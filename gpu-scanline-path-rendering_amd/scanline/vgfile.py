@@ -38,7 +38,7 @@ from fractions import Fraction
 import numpy as np
 
 from . import _abi
-from .scene import FT_GRADIENT, VGContainer
+from .scene import FT_GRADIENT, ST_NONE, VGContainer, rgba
 
 NONRATIONAL_W = 0x7F7FFFFF  # arc weight word of lines, quadratics and cubics
 _NVERT = {_abi.CURVE_LINEAR: 2, _abi.CURVE_QUADRATIC: 3, _abi.CURVE_CUBIC: 4, _abi.CURVE_RATIONAL: 3}
@@ -125,6 +125,8 @@ def _container(f, gradients):
     vg._color = f["p_color"].astype(np.int64).tolist()
     vg._ftype = ftype.astype(int).tolist()
     vg._findex = f["p_findex"].astype(np.int64).tolist()
+    vg._stype = [ST_NONE] * n_path  # the files hold no stroke fields
+    vg._scolor = [rgba(0, 0, 0)] * n_path
     vg._cpos = vpos.tolist()
     vg._ctype = ctype.astype(int).tolist()
     vg._cpath = np.repeat(con_owner, ccn).tolist()

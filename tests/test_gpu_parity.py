@@ -227,3 +227,14 @@ def test_scene_file_round_trip(rasterizer, tmp_path, suffix):
         img = rasterizer.output().copy()
         render_gpu(rasterizer, a, 320, 240, 16)
         np.testing.assert_array_equal(rasterizer.output(), img)
+
+
+@pytest.mark.parametrize("spp", [8, 32])
+def test_svg_sample_scene(rasterizer, spp):
+    """A scene read by the SVG loader (tests/golden/sample.svg: every path command, group
+    transforms, colour forms, a polygon, arcs, a stroke colour over a gradient fill)."""
+    from scanline import svg
+    vg = svg.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sample.svg"))
+    cnt, exact = compare(rasterizer, vg, vg.width, vg.height, spp, label="svg sample")
+    assert cnt["curve_fragment"] > 0
+    assert exact == 1.0
