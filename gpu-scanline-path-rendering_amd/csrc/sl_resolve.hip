@@ -121,11 +121,14 @@ __device__ __forceinline__ bool sliced_ok(int g, int8_t w) {
 template <int NW>
 struct Sliced {
     uint32_t d[4][NW];
-    __device__ __forceinline__ void init() {
+    // d = +1 or -1 (1111 in four planes) at the set bits of one member
+    __device__ __forceinline__ void init_one(const uint32_t *m, bool neg) {
 #pragma unroll
-        for (int p = 0; p < 4; ++p)
+        for (int k = 0; k < NW; ++k) {
+            d[0][k] = m[k];
 #pragma unroll
-            for (int k = 0; k < NW; ++k) d[p][k] = 0u;
+            for (int p = 1; p < 4; ++p) d[p][k] = neg ? m[k] : 0u;
+        }
     }
     // d += 1 (or -= 1) at the set bits: ripple carry (borrow) through the planes
     __device__ __forceinline__ void add(const uint32_t *m, bool neg) {
@@ -259,14 +262,19 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
 __device__ __forceinline__ int scan_add(int x) { return dpp_scan(x, 0, OpAdd()); }
 __device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, 63); }
 
-// Merged mask of the cell group at sorted positions [p, e) (records staged in LDS).
+// Merged mask of the cell group at sorted positions [p, e) (records staged in LDS).  The
+// first member is the head lane's own record i0 (meta m0): most groups have no other.
 template <int NW>
 __device__ __forceinline__ void group_mask(const uint32_t *order, const uint32_t *meta, const uint32_t *msk, int p,
-                                           int e, int8_t w_at, int rule, uint32_t *res) {
+                                           int e, uint32_t i0, uint32_t m0, int8_t w_at, int rule, uint32_t *res) {
+    uint32_t first[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) first[k] = msk[i0 * NW + k];
+    const bool neg0 = ((m0 >> 24) & 1u) != 0u;
     if (sliced_ok(e - p, w_at)) {
         Sliced<NW> st;
-        st.init();
-        for (int q = p; q < e; ++q) {
+        st.init_one(first, neg0);
+        for (int q = p + 1; q < e; ++q) {
             const uint32_t iq = order[q];
             st.add(msk + iq * NW, ((meta[iq] >> 24) & 1u) != 0u);
         }
@@ -274,7 +282,8 @@ __device__ __forceinline__ void group_mask(const uint32_t *order, const uint32_t
     } else {
         Stencil<NW> st;
         st.init(w_at);
-        for (int q = p; q < e; ++q) {
+        st.add(first, neg0);
+        for (int q = p + 1; q < e; ++q) {
             const uint32_t iq = order[q];
             st.add(msk + iq * NW, ((meta[iq] >> 24) & 1u) != 0u);
         }
@@ -341,6 +350,15 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
     int icarry = 0, nmerged = 0, xmin = INT_MAX, xmax = INT_MIN;
     uint64_t tmask = 0ull;
     const size_t ib = 2 * (size_t)base;
+    // group heads of every chunk as wave masks: a group ends at the next head
+    uint64_t hm[S];
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+        const int p = c * 64 + lane;
+        bool h = false;
+        if (p < n) h = p == 0 || (meta[order[p]] & 0xFFFFu) != (meta[order[p - 1]] & 0xFFFFu);
+        hm[c] = __ballot(h);
+    }
 #pragma unroll
     for (int c = 0; c < S; ++c) {
         if (c * 64 >= n) break;
@@ -359,9 +377,15 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
         const bool merged = head && x >= 0 && x < W;
         uint32_t res[NW];
         if (merged) {
-            int e = p + 1;
-            while (e < n && (int)(meta[order[e]] & 0xFFFFu) == cx) ++e;
-            group_mask<NW>(order, meta, msk, p, e, w_at, rule, res);
+            const uint64_t later = lane < 63 ? (hm[c] & (~0ull << (lane + 1))) : 0ull;
+            int e = n;
+            if (later) {
+                e = c * 64 + (int)__builtin_ctzll(later);
+            } else {
+                for (int c2 = c + 1; c2 < S; ++c2)
+                    if (hm[c2]) { e = c2 * 64 + (int)__builtin_ctzll(hm[c2]); break; }
+            }
+            group_mask<NW>(order, meta, msk, p, e, i, m, w_at, rule, res);
         }
         const int prev_x = 2 * prev_cx - 2;
         const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
@@ -463,7 +487,7 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
     if (merged) {
         const uint64_t later = p < 63 ? (heads & (~0ull << (p + 1))) : 0ull;
         const int e = later ? (int)__builtin_ctzll(later) : n;
-        group_mask<NW>(order, meta, msk, p, e, w_at, rule, res);
+        group_mask<NW>(order, meta, msk, p, e, i, m, w_at, rule, res);
     }
     const int prev_x = 2 * prev_cx - 2;
     const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
@@ -538,7 +562,8 @@ __device__ __forceinline__ void resolve_pack(const ResolveArgs &A, int64_t b0, i
 
     const int p = i;
     const bool v = p < n;
-    const uint32_t m = v ? meta[order[sb + p]] : 0u;
+    const uint32_t i0 = v ? order[sb + p] : 0u;
+    const uint32_t m = v ? meta[i0] : 0u;
     const int cx = (int)(m & 0xFFFFu);
     const int swn = v ? (int)(int8_t)(m >> 16) : 0;
     const int prev_cx = (v && p > 0) ? (int)(meta[order[sb + p - 1]] & 0xFFFFu) : -1;
@@ -554,7 +579,7 @@ __device__ __forceinline__ void resolve_pack(const ResolveArgs &A, int64_t b0, i
         const uint64_t segm = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << sb);
         const uint64_t later = (lane < 63 ? (heads & (~0ull << (lane + 1))) : 0ull) & segm;
         const int e = later ? (int)__builtin_ctzll(later) - sb : n;
-        group_mask<NW>(order + sb, meta, msk, p, e, w_at, rule, res);
+        group_mask<NW>(order + sb, meta, msk, p, e, i0, m, w_at, rule, res);
     }
     const int prev_x = 2 * prev_cx - 2;
     const bool span = head && p > 0 && (prev_x + 2 < x) && fill_test(rule, w_at);
@@ -598,7 +623,10 @@ __global__ void __launch_bounds__(64, 6) k_resolve_first(ResolveArgs A) {
     __shared__ uint32_t key[128], order[128], meta[128], msk[128 * NW];
     const int lane = threadIdx.x;
     const int64_t G = gridDim.x;
-    for (int64_t b0 = blockIdx.x; b0 < A.NB; b0 += 64 * G) {
+    // Strided buckets keep the waves' loads balanced (a path's neighbouring rows have
+    // similar sizes); the blocks of one XCD take consecutive b0, so neighbouring buckets,
+    // whose records share cache lines, are resolved under one L2.
+    for (int64_t b0 = xcd_chunk(blockIdx.x, (uint32_t)G); b0 < A.NB; b0 += 64 * G) {
         // The wave's next 64 buckets b0 + k G: lane k loads their offsets and scalars with
         // vector loads, read back per bucket by readlane.  (Scalar loads would share the
         // LDS wait counter, so every LDS wait of a resolve would also wait for them.)
