@@ -238,3 +238,27 @@ def test_svg_sample_scene(rasterizer, spp):
     cnt, exact = compare(rasterizer, vg, vg.width, vg.height, spp, label="svg sample")
     assert cnt["curve_fragment"] > 0
     assert exact == 1.0
+
+
+# Seeded sweep over scene kind, canvas shape (odd sizes, wide and tall), sample count,
+# colour encoding and output mode; every case is checked stage by stage against the oracle.
+SWEEP = [
+    # kind, W, H, paths, seed, spp, srgb, ms
+    (scanline.scene.KIND_CUBIC_CHAINS, 97, 61, 12, 1, 8, True, False),
+    (scanline.scene.KIND_CUBIC_CHAINS, 640, 96, 60, 2, 32, False, False),
+    (scanline.scene.KIND_MIXED, 128, 511, 90, 3, 16, True, False),
+    (scanline.scene.KIND_MIXED, 333, 129, 120, 4, 32, True, True),
+    (scanline.scene.KIND_MIXED, 64, 64, 200, 5, 8, False, True),
+    (scanline.scene.KIND_GLYPHS, 257, 255, 300, 6, 16, False, False),
+    (scanline.scene.KIND_GLYPHS, 1024, 33, 200, 7, 32, True, False),
+    (scanline.scene.KIND_MIXED, 700, 700, 400, 8, 16, True, False),
+    (scanline.scene.KIND_CUBIC_CHAINS, 2, 2, 3, 9, 32, True, False),
+    (scanline.scene.KIND_MIXED, 1, 300, 40, 10, 8, True, False),
+]
+
+
+@pytest.mark.parametrize("case", SWEEP, ids=["sweep%d" % i for i in range(len(SWEEP))])
+def test_seeded_sweep(rasterizer, case):
+    kind, W, H, n, seed, spp, srgb, ms = case
+    scene = scanline.generate(kind, W, H, n, 0x5EED0000 + seed)
+    compare(rasterizer, scene, W, H, spp, srgb=srgb, ms=ms, label="sweep seed %d" % seed)
