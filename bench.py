@@ -8,8 +8,9 @@ per pixel, coverage-alpha compositing into an sRGB8 image.  One step = one full 
 
 Prints one JSON line: Mfragments/s (raw curve fragments F per second) with Mpixels/s
 beside it, the dominant kernel's roofline entry (algorithmic bytes / its HIP-event time
-against the 8 TB/s HBM3E peak) and the CPU baseline (the OpenMP build of the oracle on a
-bounded sample of the same scene).
+against the 8 TB/s HBM3E peak, its profiled HBM traffic, and the share of the SIMDs' VALU
+issue slots its profiled instruction count takes in that time) and the CPU baseline (the
+OpenMP build of the oracle on a bounded sample of the same scene).
 """
 import argparse
 import json
@@ -54,19 +55,32 @@ STAGE_KERNEL = {("composite", False): "k_composite_cov", ("composite", True): "k
                 ("intersect", False): "k_chains", ("intersect", True): "k_chains"}
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plain_2048_32spp_traffic.md")
+INSTR_PROFILE = os.path.join(ROOT, "profiles", "plain_2048_32spp_instructions.md")
+# VALU issue ceiling: 1024 SIMDs (256 CUs x 4) at 2.4 GHz, 4 cycles per wave64 instruction
+VALU_WAVE_INSTS_PER_S = 1024 * 2.4e9 / 4
+
+
+def _profile_value(path, key, kernel):
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith(key + ":"):
+                    return json.loads(line.split(":", 1)[1]).get(kernel)
+    except OSError:
+        pass
+    return None
 
 
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` on the headline workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if absent."""
-    try:
-        with open(TRAFFIC_PROFILE) as fh:
-            for line in fh:
-                if line.startswith("kernel_traffic_bytes:"):
-                    return json.loads(line.split(":", 1)[1]).get(kernel)
-    except OSError:
-        pass
-    return None
+    return _profile_value(TRAFFIC_PROFILE, "kernel_traffic_bytes", kernel)
+
+
+def profiled_valu(kernel):
+    """Wave VALU instructions per launch of `kernel` on the headline workload, from the
+    committed rocprofv3 SQ_INSTS_VALU summary; None if absent."""
+    return _profile_value(INSTR_PROFILE, "kernel_valu_insts", kernel)
 
 
 def b_alg_total(cnt, spp, W, H):
@@ -342,9 +356,13 @@ def main():
     dom_bytes = stage_bytes(dom, cnt, args.spp, W, H, args.ms)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
     dom_kernel = STAGE_KERNEL.get((dom, args.ms), dom)
-    traffic = None
+    traffic = valu_frac = None
     if args.config == 2 and not args.ms and (W, args.paths, args.spp) == (2048, 20000, 32):
         traffic = profiled_traffic(dom_kernel)
+        valu = profiled_valu(dom_kernel)
+        if valu:
+            # issued VALU instructions over what the SIMDs can issue in the kernel's time
+            valu_frac = round(valu / (VALU_WAVE_INSTS_PER_S * stages[dom] * 1e-3), 3)
     b_alg = b_alg_total(cnt, args.spp, W, H)
 
     cpu = None
@@ -374,7 +392,7 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "alg_bytes": int(dom_bytes), "traffic": traffic},
+                     "alg_bytes": int(dom_bytes), "traffic": traffic, "valu_issue_frac": valu_frac},
         "cpu_baseline": cpu,
     }
     if cpu:
