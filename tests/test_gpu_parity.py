@@ -262,3 +262,28 @@ def test_seeded_sweep(rasterizer, case):
     kind, W, H, n, seed, spp, srgb, ms = case
     scene = scanline.generate(kind, W, H, n, 0x5EED0000 + seed)
     compare(rasterizer, scene, W, H, spp, srgb=srgb, ms=ms, label="sweep seed %d" % seed)
+
+
+def test_wide_canvas_and_long_pieces(rasterizer):
+    """Canvases wider than 64 tiles (the tile bitmaps then mark every tile) and monotone
+    pieces with more than 1024 cuts (the merge path's LDS window, then the single-thread
+    walk): a thin triangle across a 4200 x 2100 canvas, a cubic and a quadratic spanning it."""
+    W, H = 4200, 2100
+    vg = VGContainer()
+    vg.addPath(0, rgba(200, 30, 60, 200))
+    vg.addPolygon([(3.5, 5.25), (4195.0, 2090.5), (4190.0, 2060.0)])
+    vg.addPath(1, rgba(20, 120, 220, 150))
+    vg.addCubic((10.0, 2000.0), (1500.0, -900.0), (2800.0, 3000.0), (4150.0, 40.0))
+    vg.addQuadratic((4150.0, 40.0), (2000.0, 1200.0), (10.0, 2000.0))
+    vg.addPath(0, rgba(0, 0, 0, 255))
+    vg.addPolygon([(100.0, 100.0), (4100.0, 130.0), (4100.0, 131.5), (100.0, 101.0)])
+    cnt, exact = compare(rasterizer, vg, W, H, 16, label="wide canvas, long pieces")
+    assert cnt["curve_fragment"] > 10000
+    assert exact == 1.0
+
+
+def test_wide_sweep(rasterizer):
+    scene = scanline.generate(scanline.scene.KIND_MIXED, 4500, 40, 60, 0x5EED0100)
+    compare(rasterizer, scene, 4500, 40, 8, label="4500x40 mixed")
+    scene = scanline.generate(scanline.scene.KIND_GLYPHS, 48, 4400, 80, 0x5EED0101)
+    compare(rasterizer, scene, 48, 4400, 32, label="48x4400 glyphs")
