@@ -332,10 +332,13 @@ def main():
         r.rasterize(stream)
     torch.cuda.synchronize()
     stage_sum = {}
+    frame_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r.rasterize(stream)
-        for k, v in r.stepTiming().items():
+        st = r.stepTiming()
+        frame_ms.append(sum(st.values()))
+        for k, v in st.items():
             stage_sum[k] = stage_sum.get(k, 0.0) + v
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
@@ -386,6 +389,8 @@ def main():
                    "width": W, "height": H, "paths": args.paths, "spp": args.spp,
                    "output": "multisample" if args.ms else "coverage", "srgb": True},
         "mpixels_per_s": round(W * H / (ms_step * 1e-3) / 1e6, 3),
+        # HIP-event frame time (sum of the stage events of one frame), median over the steps
+        "frame_event_ms_median": round(float(np.median(frame_ms)), 4),
         "counts": cnt,
         "b_alg_bytes": int(b_alg),
         "whole_frame_gbs": round(b_alg / (ms_step * 1e-3) / 1e9, 2),
