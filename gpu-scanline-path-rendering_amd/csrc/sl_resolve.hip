@@ -418,7 +418,7 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
     __syncthreads();
 }
 
-constexpr int kResolveBlocks = 16384;
+constexpr int kResolveBlocks = 65536;
 
 // One small bucket's records, loaded a bucket ahead of its resolve (the grid-stride loop
 // keeps the next bucket's records in flight while the current one resolves); the bucket
