@@ -923,8 +923,11 @@ void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s) {
                            (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
                            (const uint32_t *)f.pcnt, (const uint32_t *)f.chains, (const uint32_t *)f.plist,
                            (const uint32_t *)cnt, chain_blocks, f.W, f.H, f.chain_t, f.ts);
+    // one block per long piece up to 32768 blocks: the pieces' sizes vary widely, and
+    // blocks that find no piece exit at once
     if (npieces)
-        hipLaunchKernelGGL(k_merge_long, dim3(4096), dim3(64), 0, s, sc, (const float2 *)f.tvtx, (const float4 *)f.ccuts,
+        hipLaunchKernelGGL(k_merge_long, dim3(npieces < 32768u ? npieces : 32768u), dim3(64), 0, s, sc,
+                           (const float2 *)f.tvtx, (const float4 *)f.ccuts,
                            (const uint32_t *)f.coff, (const uint32_t *)f.pcnt, (const float *)f.chain_t,
                            (const uint32_t *)f.plist, (const uint32_t *)(cnt + kLongPieceClass), f.ts);
 }
