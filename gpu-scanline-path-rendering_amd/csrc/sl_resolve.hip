@@ -322,11 +322,13 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
 #pragma unroll
     for (int s = 0; s < S; ++s) rk[s] = 0;
     if (S == 1) {
+#pragma unroll 8
         for (int j = 0; j < n; ++j) {
             const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)mykey[0], j);
             rk[0] += (kj < mykey[0]) ? 1 : 0;
         }
     } else {
+#pragma unroll 8
         for (int j = 0; j < n; ++j) {
             const uint32_t kj = key[j];
 #pragma unroll
@@ -458,6 +460,7 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
         mykey = ((in.meta & 0xFFFFu) << 10) | (uint32_t)lane;
     }
     int rk = 0;
+#pragma unroll 8
     for (int j = 0; j < n; ++j) {
         const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)mykey, j);
         rk += (kj < mykey) ? 1 : 0;
