@@ -66,6 +66,9 @@ struct sl_handle {
 
     // scene (device)
     DevBuf vtx, vpath, cvpos, ctype, cpath, cw, crev, cinfo, frule, finfo;
+    // pinned host staging of the scene upload, kept and grown across sl_add_vg calls
+    void *stage = nullptr;
+    size_t stage_cap = 0;
     // tables (device)
     DevBuf ptable[3];
     DevBuf ctab[2];  // [srgb] : dec_c, thr_c, dec_a, thr_a, lut_c, lut_a packed
@@ -537,6 +540,7 @@ int sl_destroy(sl_handle *h) {
                       &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp,
                       &h->b_rrec, &h->b_btile, &h->b_rtile, &h->b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
     for (DevBuf *b : bufs) release(*b);
+    if (h->stage) (void)hipHostFree(h->stage);
     if (h->events_ready)
         for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(h->ev[i]);
     delete h;
@@ -586,10 +590,35 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
         (P && (!s->fill_rule || !s->fill_color)))
         return fail(h, SL_E_ARG, "scene array pointer is NULL");
     if (P == 0 && C != 0) return fail(h, SL_E_ARG, "curves without paths");
-    std::vector<float> vtx(s->vertex, s->vertex + 2 * (size_t)V);
-    std::vector<uint32_t> vpath(V, 0xFFFFFFFFu);
-    std::vector<uint8_t> crev(C, 0);
-    std::vector<float> cw(C, -1.f);
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    // The arrays are packed into one pinned staging block (no page faults after the first
+    // scene of a size, DMA without the runtime's bounce copies), then copied asynchronously.
+    auto al = [](size_t x) { return (x + 63) & ~(size_t)63; };
+    const size_t o_vtx = 0, o_vpath = o_vtx + al(8 * (size_t)V), o_cvpos = o_vpath + al(4 * (size_t)V),
+                 o_cpath = o_cvpos + al(4 * (size_t)C), o_cw = o_cpath + al(4 * (size_t)C),
+                 o_cinfo = o_cw + al(4 * (size_t)C), o_finfo = o_cinfo + al(16 * (size_t)C),
+                 o_ctype = o_finfo + al(4 * (size_t)P), o_crev = o_ctype + al(C), o_frule = o_crev + al(C),
+                 total = o_frule + al(P) + 64;
+    if (h->stage_cap < total) {
+        if (h->stage) (void)hipHostFree(h->stage);
+        h->stage = nullptr;
+        h->stage_cap = 0;
+        const size_t want = total + total / 4;
+        if (hipHostMalloc(&h->stage, want, hipHostMallocDefault) != hipSuccess) {
+            h->stage = nullptr;
+            return fail(h, SL_E_HIP, "hipHostMalloc of %zu bytes failed", want);
+        }
+        h->stage_cap = want;
+    }
+    uint8_t *st = (uint8_t *)h->stage;
+    float *vtx = (float *)(st + o_vtx);
+    uint32_t *vpath = (uint32_t *)(st + o_vpath);
+    uint8_t *crev = st + o_crev;
+    float *cw = (float *)(st + o_cw);
+    if (V) std::memcpy(vtx, s->vertex, 8 * (size_t)V);
+    std::fill(vpath, vpath + V, 0xFFFFFFFFu);
+    std::fill(crev, crev + C, (uint8_t)0);
+    std::fill(cw, cw + C, -1.f);
     uint32_t last_path = 0;
     for (uint32_t c = 0; c < C; ++c) {
         const uint8_t t = s->curve_type[c];
@@ -623,7 +652,7 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
     }
     for (uint32_t i = 0; i < V; ++i)
         if (vpath[i] == 0xFFFFFFFFu) vpath[i] = 0;  // vertices of no curve: harmless, never read
-    std::vector<uint32_t> cinfo((size_t)C * 4);
+    uint32_t *cinfo = (uint32_t *)(st + o_cinfo);
     for (uint32_t c = 0; c < C; ++c) {
         uint32_t wb;
         std::memcpy(&wb, &cw[c], 4);
@@ -632,20 +661,31 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
         cinfo[4 * (size_t)c + 2] = (uint32_t)s->curve_type[c] | ((uint32_t)crev[c] << 8);
         cinfo[4 * (size_t)c + 3] = wb;
     }
-    std::vector<uint32_t> finfo(P);
+    uint32_t *finfo = (uint32_t *)(st + o_finfo);
     for (uint32_t p = 0; p < P; ++p) {
         const uint32_t c = s->fill_color[p];
         finfo[p] = (c & 0xFF000000u) ? c : 0u;  // ras_base.cpp:147-151
     }
-    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
-    int rc;
-    if ((rc = upload(h, h->vtx, vtx.data(), vtx.size() * 4)) || (rc = upload(h, h->vpath, vpath.data(), (size_t)V * 4)) ||
-        (rc = upload(h, h->cvpos, s->curve_vertex_pos, (size_t)C * 4)) || (rc = upload(h, h->ctype, s->curve_type, C)) ||
-        (rc = upload(h, h->cpath, s->curve_path_id, (size_t)C * 4)) || (rc = upload(h, h->cw, cw.data(), (size_t)C * 4)) ||
-        (rc = upload(h, h->crev, crev.data(), C)) || (rc = upload(h, h->cinfo, cinfo.data(), (size_t)C * 16)) ||
-        (rc = upload(h, h->frule, s->fill_rule, P)) ||
-        (rc = upload(h, h->finfo, finfo.data(), (size_t)P * 4)))
-        return rc;
+    if (C) {
+        std::memcpy(st + o_cvpos, s->curve_vertex_pos, 4 * (size_t)C);
+        std::memcpy(st + o_cpath, s->curve_path_id, 4 * (size_t)C);
+        std::memcpy(st + o_ctype, s->curve_type, C);
+    }
+    if (P) std::memcpy(st + o_frule, s->fill_rule, P);
+    struct Part { DevBuf *b; size_t off, bytes; };
+    const Part parts[] = {{&h->vtx, o_vtx, 8 * (size_t)V},       {&h->vpath, o_vpath, 4 * (size_t)V},
+                          {&h->cvpos, o_cvpos, 4 * (size_t)C},   {&h->ctype, o_ctype, C},
+                          {&h->cpath, o_cpath, 4 * (size_t)C},   {&h->cw, o_cw, 4 * (size_t)C},
+                          {&h->crev, o_crev, C},                 {&h->cinfo, o_cinfo, 16 * (size_t)C},
+                          {&h->frule, o_frule, P},               {&h->finfo, o_finfo, 4 * (size_t)P}};
+    for (const Part &q : parts) {
+        int rc = ensure(h, *q.b, q.bytes);
+        if (rc) return rc;
+    }
+    // the null stream orders these copies after earlier work as hipMemcpy does
+    for (const Part &q : parts)
+        if (q.bytes) HIPCHK(h, hipMemcpyAsync(q.b->p, st + q.off, q.bytes, hipMemcpyHostToDevice, nullptr));
+    HIPCHK(h, hipStreamSynchronize(nullptr));
     h->P = P; h->C = C; h->V = (P == 0) ? 0 : V;
     h->has_scene = true;
     return SL_OK;

@@ -287,3 +287,33 @@ def test_wide_sweep(rasterizer):
     compare(rasterizer, scene, 4500, 40, 8, label="4500x40 mixed")
     scene = scanline.generate(scanline.scene.KIND_GLYPHS, 48, 4400, 80, 0x5EED0101)
     compare(rasterizer, scene, 48, 4400, 32, label="48x4400 glyphs")
+
+
+def test_add_vg_rejects_invalid_scenes(rasterizer):
+    """sl_add_vg validates the scene before anything is uploaded: each malformed scene is
+    refused with SL_E_ARG and a message naming the first offending curve (the reference
+    would read out of bounds instead), and a valid scene loads after the refusals."""
+    from scanline.scene import SceneArrays
+
+    def scene(cpos, ctype, cpath, nv=8, paths=2):
+        v = np.arange(2 * nv, dtype=np.float32).reshape(-1, 2)
+        return SceneArrays(v, cpos, ctype, cpath, [-1.0] * len(ctype), [0] * paths,
+                           [rgba(1, 2, 3)] * paths)
+
+    L, Q = scanline.CURVE_LINEAR, scanline.CURVE_QUADRATIC
+    bad = [
+        (scene([0, 2], [L, 0x07], [0, 1]), "invalid type"),
+        (scene([0, 2], [L, L], [0, 2]), "path id"),
+        (scene([0, 2], [L, L], [1, 0]), "non-decreasing"),
+        (scene([0, 7], [L, Q], [0, 1]), "exceed n_vertices"),
+        (scene([0, 1], [L, L], [0, 1]), "belongs to two curves"),
+    ]
+    rasterizer.setSamples(8)
+    rasterizer.resizeOutputBuffer(16, 16)
+    for s, msg in bad:
+        with pytest.raises(scanline.ScanlineError) as e:
+            rasterizer.addVg(s)
+        assert msg in str(e.value), (msg, str(e.value))
+        assert "curve 1" in str(e.value) or "vertex 1" in str(e.value), str(e.value)
+    rasterizer.addVg(scene([0, 2], [L, Q], [0, 1]))
+    rasterizer.rasterize()
