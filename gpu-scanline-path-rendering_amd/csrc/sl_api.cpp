@@ -15,6 +15,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/scanline.h"
@@ -615,52 +616,121 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
     uint32_t *vpath = (uint32_t *)(st + o_vpath);
     uint8_t *crev = st + o_crev;
     float *cw = (float *)(st + o_cw);
-    if (V) std::memcpy(vtx, s->vertex, 8 * (size_t)V);
-    std::fill(vpath, vpath + V, 0xFFFFFFFFu);
-    std::fill(crev, crev + C, (uint8_t)0);
-    std::fill(cw, cw + C, -1.f);
-    uint32_t last_path = 0;
-    for (uint32_t c = 0; c < C; ++c) {
-        const uint8_t t = s->curve_type[c];
-        if (t != SL_CURVE_LINEAR && t != SL_CURVE_QUADRATIC && t != SL_CURVE_CUBIC && t != SL_CURVE_RATIONAL)
-            return fail(h, SL_E_ARG, "curve %u has invalid type 0x%x", c, t);
-        const uint32_t p = s->curve_path_id[c];
-        if (p >= P) return fail(h, SL_E_ARG, "curve %u has path id %u >= n_paths %u", c, p, P);
-        if (p < last_path) return fail(h, SL_E_ARG, "curve %u: curve_path_id must be non-decreasing", c);
-        last_path = p;
-        const uint32_t vp = s->curve_vertex_pos[c];
-        const int nv = t & 7;
-        if ((uint64_t)vp + nv > V) return fail(h, SL_E_ARG, "curve %u vertices [%u, %u) exceed n_vertices %u", c, vp, vp + nv, V);
-        for (int k = 0; k < nv; ++k) {
-            if (vpath[vp + k] != 0xFFFFFFFFu) return fail(h, SL_E_ARG, "vertex %u belongs to two curves", vp + k);
-            vpath[vp + k] = p;
+    // Validation and packing run over chunks of curves on host threads.  Errors are the
+    // ones a curve-by-curve pass would report first: the lowest failing curve, and a vertex
+    // shared by two curves is reported at the later curve (vertices go to their lowest
+    // curve by an atomic minimum, then every curve checks that it owns its vertices).
+    uint32_t *owner = vpath;  // curve owning each vertex; turned into path ids at the end
+    uint32_t *cinfo = (uint32_t *)(st + o_cinfo);
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned T = (C >= 65536u && hw > 1) ? hw : 1u;
+    auto for_chunks = [&](size_t n, auto &&body) {
+        // body(t, lo, hi) on T contiguous chunks of [0, n)
+        if (T == 1 || n < 4096) { body(0u, (size_t)0, n); return; }
+        std::vector<std::thread> th;
+        th.reserve(T);
+        for (unsigned t = 0; t < T; ++t)
+            th.emplace_back([&, t] { body(t, n * t / T, n * (t + 1) / T); });
+        for (auto &x : th) x.join();
+    };
+    struct Err { uint32_t at = 0xFFFFFFFFu; int kind = 0; uint32_t a = 0, b = 0; };
+    std::vector<Err> errs(T);
+    auto better = [](const Err &x, const Err &y) { return x.at < y.at; };
+    if (V) {
+        for_chunks(2 * (size_t)V, [&](unsigned, size_t lo, size_t hi) {
+            std::memcpy(vtx + lo, s->vertex + lo, (hi - lo) * 4);
+        });
+        for_chunks(V, [&](unsigned, size_t lo, size_t hi) { std::fill(owner + lo, owner + hi, 0xFFFFFFFFu); });
+    }
+    // structural checks, in the order a curve-by-curve pass applies them
+    for_chunks(C, [&](unsigned t, size_t lo, size_t hi) {
+        for (size_t c = lo; c < hi; ++c) {
+            const uint8_t ty = s->curve_type[c];
+            Err e;
+            const uint32_t p = s->curve_path_id[c];
+            const uint32_t vp = s->curve_vertex_pos[c];
+            if (ty != SL_CURVE_LINEAR && ty != SL_CURVE_QUADRATIC && ty != SL_CURVE_CUBIC && ty != SL_CURVE_RATIONAL)
+                e = {(uint32_t)c, 1, ty, 0};
+            else if (p >= P)
+                e = {(uint32_t)c, 2, p, 0};
+            else if (c > 0 && p < s->curve_path_id[c - 1])
+                e = {(uint32_t)c, 3, 0, 0};
+            else if ((uint64_t)vp + (ty & 7) > V)
+                e = {(uint32_t)c, 4, vp, vp + (ty & 7)};
+            if (e.kind) { errs[t] = e; return; }
         }
-        if (t == SL_CURVE_RATIONAL) cw[c] = s->curve_arc_w ? s->curve_arc_w[c] : 1.f;
-        // unifyCurveVertexOrder (ras_base.cpp:523-569): order by float bit patterns
-        uint32_t ixf, ixl, iyf, iyl;
-        std::memcpy(&ixf, &vtx[2 * (size_t)vp], 4);
-        std::memcpy(&iyf, &vtx[2 * (size_t)vp + 1], 4);
-        std::memcpy(&ixl, &vtx[2 * (size_t)(vp + nv - 1)], 4);
-        std::memcpy(&iyl, &vtx[2 * (size_t)(vp + nv - 1) + 1], 4);
-        if (!(iyf < iyl || (iyf == iyl && ixf < ixl))) {
-            crev[c] = 1;
-            for (int k = 0; k < nv / 2; ++k) {
-                std::swap(vtx[2 * (size_t)(vp + k)], vtx[2 * (size_t)(vp + nv - 1 - k)]);
-                std::swap(vtx[2 * (size_t)(vp + k) + 1], vtx[2 * (size_t)(vp + nv - 1 - k) + 1]);
+    });
+    Err first = *std::min_element(errs.begin(), errs.end(), better);
+    // A decreasing path id right after an earlier failure is that failure's to report; the
+    // chunk scan compares with the previous curve's id, a serial pass with the last good
+    // one, and both agree up to the first failing curve.
+    const uint32_t nchecked = first.kind ? first.at : C;
+    // vertex ownership among the curves before the first structural failure
+    for_chunks(nchecked, [&](unsigned, size_t lo, size_t hi) {
+        for (size_t c = lo; c < hi; ++c) {
+            const uint32_t vp = s->curve_vertex_pos[c];
+            const int nv = s->curve_type[c] & 7;
+            for (int k = 0; k < nv; ++k) {
+                uint32_t cur = __atomic_load_n(&owner[vp + k], __ATOMIC_RELAXED);
+                while ((uint32_t)c < cur &&
+                       !__atomic_compare_exchange_n(&owner[vp + k], &cur, (uint32_t)c, true, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED)) {
+                }
             }
         }
+    });
+    std::vector<Err> cerrs(T);
+    for_chunks(nchecked, [&](unsigned t, size_t lo, size_t hi) {
+        for (size_t c = lo; c < hi; ++c) {
+            const uint32_t vp = s->curve_vertex_pos[c];
+            const int nv = s->curve_type[c] & 7;
+            for (int k = 0; k < nv; ++k)
+                if (owner[vp + k] != (uint32_t)c) { cerrs[t] = {(uint32_t)c, 5, vp + (uint32_t)k, 0}; return; }
+        }
+    });
+    const Err shared = *std::min_element(cerrs.begin(), cerrs.end(), better);
+    if (shared.kind && shared.at < first.at) first = shared;
+    switch (first.kind) {
+    case 1: return fail(h, SL_E_ARG, "curve %u has invalid type 0x%x", first.at, first.a);
+    case 2: return fail(h, SL_E_ARG, "curve %u has path id %u >= n_paths %u", first.at, first.a, P);
+    case 3: return fail(h, SL_E_ARG, "curve %u: curve_path_id must be non-decreasing", first.at);
+    case 4: return fail(h, SL_E_ARG, "curve %u vertices [%u, %u) exceed n_vertices %u", first.at, first.a, first.b, V);
+    case 5: return fail(h, SL_E_ARG, "vertex %u belongs to two curves", first.a);
+    default: break;
     }
-    for (uint32_t i = 0; i < V; ++i)
-        if (vpath[i] == 0xFFFFFFFFu) vpath[i] = 0;  // vertices of no curve: harmless, never read
-    uint32_t *cinfo = (uint32_t *)(st + o_cinfo);
-    for (uint32_t c = 0; c < C; ++c) {
-        uint32_t wb;
-        std::memcpy(&wb, &cw[c], 4);
-        cinfo[4 * (size_t)c] = s->curve_vertex_pos[c];
-        cinfo[4 * (size_t)c + 1] = s->curve_path_id[c];
-        cinfo[4 * (size_t)c + 2] = (uint32_t)s->curve_type[c] | ((uint32_t)crev[c] << 8);
-        cinfo[4 * (size_t)c + 3] = wb;
-    }
+    // orientation, weights and the packed curve words; every vertex now has one curve
+    for_chunks(C, [&](unsigned, size_t lo, size_t hi) {
+        for (size_t c = lo; c < hi; ++c) {
+            const uint8_t ty = s->curve_type[c];
+            const uint32_t vp = s->curve_vertex_pos[c];
+            const int nv = ty & 7;
+            cw[c] = (ty == SL_CURVE_RATIONAL) ? (s->curve_arc_w ? s->curve_arc_w[c] : 1.f) : -1.f;
+            // unifyCurveVertexOrder (ras_base.cpp:523-569): order by float bit patterns
+            uint32_t ixf, ixl, iyf, iyl;
+            std::memcpy(&ixf, &vtx[2 * (size_t)vp], 4);
+            std::memcpy(&iyf, &vtx[2 * (size_t)vp + 1], 4);
+            std::memcpy(&ixl, &vtx[2 * (size_t)(vp + nv - 1)], 4);
+            std::memcpy(&iyl, &vtx[2 * (size_t)(vp + nv - 1) + 1], 4);
+            crev[c] = 0;
+            if (!(iyf < iyl || (iyf == iyl && ixf < ixl))) {
+                crev[c] = 1;
+                for (int k = 0; k < nv / 2; ++k) {
+                    std::swap(vtx[2 * (size_t)(vp + k)], vtx[2 * (size_t)(vp + nv - 1 - k)]);
+                    std::swap(vtx[2 * (size_t)(vp + k) + 1], vtx[2 * (size_t)(vp + nv - 1 - k) + 1]);
+                }
+            }
+            uint32_t wb;
+            std::memcpy(&wb, &cw[c], 4);
+            cinfo[4 * c] = vp;
+            cinfo[4 * c + 1] = s->curve_path_id[c];
+            cinfo[4 * c + 2] = (uint32_t)ty | ((uint32_t)crev[c] << 8);
+            cinfo[4 * c + 3] = wb;
+        }
+    });
+    // owning path of each vertex; vertices of no curve: harmless, never read
+    for_chunks(V, [&](unsigned, size_t lo, size_t hi) {
+        for (size_t v = lo; v < hi; ++v) owner[v] = owner[v] == 0xFFFFFFFFu ? 0u : s->curve_path_id[owner[v]];
+    });
     uint32_t *finfo = (uint32_t *)(st + o_finfo);
     for (uint32_t p = 0; p < P; ++p) {
         const uint32_t c = s->fill_color[p];

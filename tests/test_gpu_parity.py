@@ -317,3 +317,46 @@ def test_add_vg_rejects_invalid_scenes(rasterizer):
         assert "curve 1" in str(e.value) or "vertex 1" in str(e.value), str(e.value)
     rasterizer.addVg(scene([0, 2], [L, Q], [0, 1]))
     rasterizer.rasterize()
+
+
+def test_add_vg_validation_large_scene(rasterizer):
+    """The same refusals on scenes large enough for the threaded validation: the reported
+    curve is the first failing one in curve order, whichever chunk finds it."""
+    from scanline.scene import SceneArrays
+    n = 120000
+    L = scanline.CURVE_LINEAR
+
+    def scene(mod=None):
+        cpos = np.arange(n, dtype=np.uint32) * 2
+        ctype = np.full(n, L, dtype=np.uint8)
+        cpath = (np.arange(n, dtype=np.uint32) // 4)
+        if mod:
+            mod(cpos, ctype, cpath)
+        v = np.random.default_rng(1).uniform(0, 60, (2 * n, 2)).astype(np.float32)
+        return SceneArrays(v, cpos, ctype, cpath, np.full(n, -1, np.float32), np.zeros(n // 4, np.uint8),
+                           np.full(n // 4, rgba(9, 9, 9, 90), np.uint32))
+
+    def shared(cpos, ctype, cpath):
+        cpos[100001] = cpos[7] + 1         # shares a vertex with curve 7: reported at 100001
+    def type_then_shared(cpos, ctype, cpath):
+        ctype[90000] = 0x05
+        cpos[100001] = cpos[7] + 1
+    def shared_then_type(cpos, ctype, cpath):
+        cpos[50001] = cpos[50000] + 1      # reported at 50001, before the bad type
+        ctype[110000] = 0x05
+    def order(cpos, ctype, cpath):
+        cpath[70000] = 0
+
+    cases = [(shared, "vertex %d belongs to two curves" % (2 * 7 + 1)),
+             (type_then_shared, "curve 90000 has invalid type 0x5"),
+             (shared_then_type, "vertex %d belongs to two curves" % (2 * 50000 + 1)),
+             (order, "curve 70000: curve_path_id must be non-decreasing")]
+    rasterizer.setSamples(8)
+    rasterizer.resizeOutputBuffer(64, 64)
+    for mod, msg in cases:
+        with pytest.raises(scanline.ScanlineError) as e:
+            rasterizer.addVg(scene(mod))
+        assert msg in str(e.value), (msg, str(e.value))
+    good = scene()
+    cnt, exact = compare(rasterizer, good, 64, 64, 8, label="large valid scene")
+    assert exact == 1.0
