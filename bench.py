@@ -179,6 +179,7 @@ def bench_batch(args):
         "config": {"workload": "batched: 4096 scenes, sides log-uniform 16-512, 5-200 cubic chains, 8/16 spp",
                    "scenes": len(scenes), "pixels": int(sum(w * h for w, h in sizes))},
         "counts": cnt,
+        "count_pixel": count_pixel,
         "mfragments_per_s": round(F / (batch_ms * 1e-3) / 1e6, 2),
         "per_scene_loop_ms": round(loop_ms, 3),
         "speedup_vs_loop": round(loop_ms / batch_ms, 2),
@@ -353,6 +354,14 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
 
+    # the reference's count-pixel (sum over draw items of 2 * width), from one extra frame
+    # with the parity dumps on, outside the timed region
+    r.dumpDebugData(True)
+    r.rasterize(stream)
+    torch.cuda.synchronize()
+    count_pixel = int(r.pixelCount())
+    r.dumpDebugData(False)
+
     F = cnt["curve_fragment"]
     value = F / (ms_step * 1e-3) / 1e6
     dom = max((k for k in stages if stage_bytes(k, cnt, args.spp, W, H, args.ms) > 0), key=lambda k: stages[k])
@@ -392,6 +401,7 @@ def main():
         # HIP-event frame time (sum of the stage events of one frame), median over the steps
         "frame_event_ms_median": round(float(np.median(frame_ms)), 4),
         "counts": cnt,
+        "count_pixel": count_pixel,
         "b_alg_bytes": int(b_alg),
         "whole_frame_gbs": round(b_alg / (ms_step * 1e-3) / 1e9, 2),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
