@@ -12,7 +12,9 @@
  * Threading: a handle is single-threaded (like the reference's class, which keeps
  * static state in rasterizeImpl, ras_cut_mask_comb_scanline.cu:882-884); separate
  * handles are independent.  All GPU work of a handle runs on the HIP stream passed to
- * sl_rasterize (NULL = the default stream).
+ * sl_rasterize (NULL = the default stream).  sl_add_vg validates and packs large scenes
+ * on up to 16 host worker threads of its own, copies on the default stream and returns
+ * once the copies are done.
  */
 #ifndef SCANLINE_MI355X_H
 #define SCANLINE_MI355X_H
