@@ -627,11 +627,19 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
     auto for_chunks = [&](size_t n, auto &&body) {
         // body(t, lo, hi) on T contiguous chunks of [0, n)
         if (T == 1 || n < 4096) { body(0u, (size_t)0, n); return; }
-        std::vector<std::thread> th;
-        th.reserve(T);
+        std::thread th[16];
+        bool started[16] = {};
+        for (unsigned t = 0; t < T; ++t) {
+            const size_t lo = n * t / T, hi = n * (t + 1) / T;
+            try {
+                th[t] = std::thread([&body, t, lo, hi] { body(t, lo, hi); });
+                started[t] = true;
+            } catch (...) {
+                body(t, lo, hi);  // no thread to be had (limits): this chunk runs here
+            }
+        }
         for (unsigned t = 0; t < T; ++t)
-            th.emplace_back([&, t] { body(t, n * t / T, n * (t + 1) / T); });
-        for (auto &x : th) x.join();
+            if (started[t]) th[t].join();
     };
     struct Err { uint32_t at = 0xFFFFFFFFu; int kind = 0; uint32_t a = 0, b = 0; };
     std::vector<Err> errs(T);
