@@ -179,7 +179,6 @@ def bench_batch(args):
         "config": {"workload": "batched: 4096 scenes, sides log-uniform 16-512, 5-200 cubic chains, 8/16 spp",
                    "scenes": len(scenes), "pixels": int(sum(w * h for w, h in sizes))},
         "counts": cnt,
-        "count_pixel": count_pixel,
         "mfragments_per_s": round(F / (batch_ms * 1e-3) / 1e6, 2),
         "per_scene_loop_ms": round(loop_ms, 3),
         "speedup_vs_loop": round(loop_ms / batch_ms, 2),
