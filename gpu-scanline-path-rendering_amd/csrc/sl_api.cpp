@@ -87,6 +87,9 @@ struct sl_handle {
     std::vector<uint32_t> d_binfo, d_bicnt, d_bext, d_boff, d_rowkeys, d_rowvals, d_rowbeg, d_rowend;
     hipEvent_t ev[kStages + 1] = {};
     bool events_ready = false;
+    // a replayed frame (see sl_rasterize) is still running until done_ev has passed
+    hipEvent_t done_ev = nullptr;
+    bool pending = false;
     int64_t NB = 0;
     uint32_t *nz_vals_sorted = nullptr;
 };
@@ -519,9 +522,15 @@ int sl_create(int device, sl_handle **out) {
     h->device = device;
     for (int i = 0; i <= kStages; ++i) {
         if (hipEventCreate(&h->ev[i]) != hipSuccess) {
+            for (int k = 0; k < i; ++k) (void)hipEventDestroy(h->ev[k]);
             delete h;
             return SL_E_HIP;
         }
+    }
+    if (hipEventCreateWithFlags(&h->done_ev, hipEventDisableTiming) != hipSuccess) {
+        for (int k = 0; k <= kStages; ++k) (void)hipEventDestroy(h->ev[k]);
+        delete h;
+        return SL_E_HIP;
     }
     h->events_ready = true;
     *out = h;
@@ -542,8 +551,10 @@ int sl_destroy(sl_handle *h) {
                       &h->b_rrec, &h->b_btile, &h->b_rtile, &h->b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
     for (DevBuf *b : bufs) release(*b);
     if (h->stage) (void)hipHostFree(h->stage);
-    if (h->events_ready)
+    if (h->events_ready) {
         for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(h->ev[i]);
+        (void)hipEventDestroy(h->done_ev);
+    }
     delete h;
     return SL_OK;
 }
@@ -553,24 +564,28 @@ int sl_set_samples(sl_handle *h, int samples) {
     if (samples != 8 && samples != 16 && samples != 32)
         return fail(h, SL_E_UNSUPPORTED, "samples must be 8, 16 or 32 (got %d)", samples);
     h->spp = samples;
+    h->sized = false;
     return SL_OK;
 }
 
 int sl_set_multisample_output(sl_handle *h, int enable) {
     if (!h) return SL_E_ARG;
     h->ms = enable ? 1 : 0;
+    h->sized = false;
     return SL_OK;
 }
 
 int sl_enable_srgb_correction(sl_handle *h, int enable) {
     if (!h) return SL_E_ARG;
     h->srgb = enable ? 1 : 0;
+    h->sized = false;
     return SL_OK;
 }
 
 int sl_set_input_transform(sl_handle *h, const float m[16]) {
     if (!h || !m) return SL_E_ARG;
     std::memcpy(h->m, m, sizeof h->m);
+    h->sized = false;
     return SL_OK;
 }
 
@@ -580,11 +595,13 @@ int sl_resize_output_buffer(sl_handle *h, int width, int height) {
         return fail(h, SL_E_ARG, "output size %dx%d outside 1..32766", width, height);
     h->W = width;
     h->H = height;
+    h->sized = false;
     return SL_OK;
 }
 
 int sl_add_vg(sl_handle *h, const sl_scene *s) {
     if (!h) return SL_E_ARG;
+    h->sized = false;
     if (!s) return fail(h, SL_E_ARG, "scene is NULL");
     const uint32_t P = s->n_paths, C = s->n_curves, V = s->n_vertices;
     if ((C && (!s->curve_vertex_pos || !s->curve_type || !s->curve_path_id)) || (V && !s->vertex) ||
@@ -773,13 +790,28 @@ int sl_clear(sl_handle *h) {
     if (!h) return SL_E_ARG;
     h->has_scene = false;
     h->P = h->C = h->V = 0;
+    h->sized = false;
     return SL_OK;
 }
 
 int sl_rasterize(sl_handle *h, void *stream) {
     if (!h) return SL_E_ARG;
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
-    return render(h, (hipStream_t)stream);
+    // Once a frame has sized the buffers, the same scene under the same settings is
+    // re-rendered without host synchronisation: the launches are those of that frame
+    // (every kernel runs again; only its counts and sizes are taken over), queued on the
+    // stream, and the call returns at once.  Step timing and parity dumps need the host
+    // reads of a full frame, so they keep every frame synchronous.
+    const hipStream_t s = (hipStream_t)stream;
+    const bool replay = h->sized && !h->timing && !h->debug;
+    const int rc = render(h, s, replay);
+    h->pending = false;
+    if (rc) return rc;
+    if (replay) {
+        HIPCHK(h, hipEventRecord(h->done_ev, s));
+        h->pending = true;
+    }
+    return SL_OK;
 }
 
 int sl_output_device_ptr(sl_handle *h, void **dev_ptr, int *pitch_bytes) {
@@ -793,6 +825,10 @@ int sl_output_device_ptr(sl_handle *h, void **dev_ptr, int *pitch_bytes) {
 int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags) {
     if (!h || !host_rgba) return SL_E_ARG;
     if (!h->b_image.p) return fail(h, SL_E_STATE, "no image rendered yet");
+    if (h->pending) {
+        HIPCHK(h, hipEventSynchronize(h->done_ev));
+        h->pending = false;
+    }
     const size_t row = (size_t)h->W * 4;
     HIPCHK(h, hipMemcpy(host_rgba, h->b_image.p, row * h->H, hipMemcpyDeviceToHost));
     if (flags & 1) flip_rows_alpha(host_rgba, h->W, h->H);
@@ -833,6 +869,7 @@ const char *sl_last_error(sl_handle *h) { return h ? h->err.c_str() : "null hand
 int sl_debug_enable(sl_handle *h, int enable) {
     if (!h) return SL_E_ARG;
     h->debug = enable != 0;
+    h->sized = false;
     return SL_OK;
 }
 

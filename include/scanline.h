@@ -128,7 +128,13 @@ int sl_clear(sl_handle *h);
 
 /* rasterize() (ras_base.cpp:1131-1195 -> rasterizeImpl<2>, .cu:879-1453): renders the
  * scene into the handle's device RGBA8 image (row 0 = y in [0,1), the reference's
- * GL framebuffer order).  `stream` is a hipStream_t or NULL. */
+ * GL framebuffer order).  `stream` is a hipStream_t or NULL.
+ * The first frame after any change (scene, transform, output size, sample count, output
+ * mode, colour encoding, debug dumps) reads its counts back and returns when the frame is
+ * done.  Later frames of the unchanged scene reuse those counts and sizes: every kernel
+ * runs again, queued on `stream` with no host synchronisation, and the call returns at
+ * once (step timing and debug dumps keep every frame synchronous).  sl_read_output waits
+ * for such a frame; users of sl_output_device_ptr order their work after it on `stream`. */
 int sl_rasterize(sl_handle *h, void *stream);
 
 /* Device pointer and row pitch (bytes) of the handle's RGBA8 output image. */

@@ -360,3 +360,50 @@ def test_add_vg_validation_large_scene(rasterizer):
     good = scene()
     cnt, exact = compare(rasterizer, good, 64, 64, 8, label="large valid scene")
     assert exact == 1.0
+
+
+def test_rerender_unchanged_scene(rasterizer):
+    """Frames after the first of an unchanged scene are queued without host reads.  Each
+    one renders the image again: it is poisoned between frames and must come back equal
+    to the oracle's.  A changed setting makes the next frame a full one again."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime libscanline.so is bound to
+    W, H, spp = 256, 192, 16
+    scene = scanline.generate(scanline.scene.KIND_MIXED, W, H, 80, 0x5EED0200)
+    r = rasterizer
+    r.setSamples(spp)
+    r.enableSRGBCorrection(True)
+    r.setMultisampleOutput(False)
+    r.resizeOutputBuffer(W, H)
+    r.setInputTransform(np.eye(4, dtype=np.float32))
+    r.dumpDebugData(False)
+    r.enableStepTiming(False)
+    r.addVg(scene)
+
+    def poison(v):
+        p, pitch = r.outputDevicePtr()
+        assert hip.hipMemset(ctypes.c_void_p(p), ctypes.c_int(v), ctypes.c_size_t(pitch * H)) == 0
+
+    def check(img, transform=None):
+        ref = pyoracle.run(scene, W, H, spp, srgb=True, transform=transform).get("image").reshape(H, W, 4)
+        assert np.abs(img.astype(np.int32) - ref.astype(np.int32)).max() <= 1
+
+    r.rasterize()
+    first = r.output()
+    check(first)
+    counts = r.elementNumber()
+    for k in range(3):
+        poison(17 + k)
+        r.rasterize()
+        np.testing.assert_array_equal(r.output(), first)
+        assert r.elementNumber() == counts
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0], m[1, 1], m[3, 0] = 0.8, 1.15, 9.5
+    r.setInputTransform(m)
+    r.rasterize()
+    moved = r.output()
+    check(moved, m)
+    assert not np.array_equal(moved, first)
+    poison(99)
+    r.rasterize()
+    np.testing.assert_array_equal(r.output(), moved)
