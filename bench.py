@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--config", type=int, choices=[2, 3], default=2,
                     help="plain mode workload: 2 = 2048^2 mixed paths (headline), 3 = text-like 4096^2")
     ap.add_argument("--batch-scenes", type=int, default=4096)
+    ap.add_argument("--no-count-pixel", action="store_true",
+                    help="skip the untimed debug frame that gives the reference's count-pixel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this renderer runs on one GPU; --gpus %d is not supported "
@@ -364,11 +366,13 @@ def main():
 
     # the reference's count-pixel (sum over draw items of 2 * width), from one extra frame
     # with the parity dumps on, outside the timed region
-    r.dumpDebugData(True)
-    r.rasterize(stream)
-    torch.cuda.synchronize()
-    count_pixel = int(r.pixelCount())
-    r.dumpDebugData(False)
+    count_pixel = None
+    if not args.no_count_pixel:
+        r.dumpDebugData(True)
+        r.rasterize(stream)
+        torch.cuda.synchronize()
+        count_pixel = int(r.pixelCount())
+        r.dumpDebugData(False)
 
     F = cnt["curve_fragment"]
     value = F / (ms_step * 1e-3) / 1e6
