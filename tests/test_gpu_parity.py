@@ -407,3 +407,39 @@ def test_rerender_unchanged_scene(rasterizer):
     poison(99)
     r.rasterize()
     np.testing.assert_array_equal(r.output(), moved)
+
+
+def test_rerender_on_a_torch_stream(rasterizer):
+    """Frames queued on a caller's own stream: the read-back waits for the frame through
+    the event recorded on that stream, and torch work queued after it sees the image."""
+    import torch
+    W, H, spp = 192, 160, 8
+    scene = scanline.generate(scanline.scene.KIND_CUBIC_CHAINS, W, H, 30, 0x5EED0300)
+    r = rasterizer
+    r.setSamples(spp)
+    r.enableSRGBCorrection(True)
+    r.setMultisampleOutput(False)
+    r.resizeOutputBuffer(W, H)
+    r.setInputTransform(np.eye(4, dtype=np.float32))
+    r.dumpDebugData(False)
+    r.enableStepTiming(False)
+    r.addVg(scene)
+    ref = pyoracle.run(scene, W, H, spp, srgb=True).get("image").reshape(H, W, 4)
+    st = torch.cuda.Stream()
+    ptr, pitch = None, None
+    with torch.cuda.stream(st):
+        for k in range(4):
+            r.rasterize(st.cuda_stream)
+            if k == 3:
+                # a device-side copy of the image, queued after the frame on the same stream
+                ptr, pitch = r.outputDevicePtr()
+                dev = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
+                import ctypes
+                hip = ctypes.CDLL("libamdhip64.so.7")
+                assert hip.hipMemcpyAsync(ctypes.c_void_p(dev.data_ptr()), ctypes.c_void_p(ptr),
+                                          ctypes.c_size_t(pitch * H), ctypes.c_int(3),
+                                          ctypes.c_void_p(st.cuda_stream)) == 0
+        img = r.output()
+    st.synchronize()
+    assert np.abs(img.astype(np.int32) - ref.astype(np.int32)).max() <= 1
+    np.testing.assert_array_equal(dev.cpu().numpy(), img)
