@@ -90,6 +90,7 @@ struct sl_handle {
     // a replayed frame (see sl_rasterize) is still running until done_ev has passed
     hipEvent_t done_ev = nullptr;
     bool pending = false;
+    hipStream_t pending_stream = nullptr;  // the stream done_ev was recorded on
     int64_t NB = 0;
     uint32_t *nz_vals_sorted = nullptr;
 };
@@ -506,6 +507,16 @@ extern "C" {
 
 const char *sl_version(void) { return "scanline-mi355x 0.1 (gfx950)"; }
 
+// Wait for a queued frame to finish: it reads the scene and writes the frame buffers, so
+// the host must not overwrite or read them before it has passed.
+static int settle(sl_handle *h) {
+    if (h->pending) {
+        HIPCHK(h, hipEventSynchronize(h->done_ev));
+        h->pending = false;
+    }
+    return SL_OK;
+}
+
 int sl_create(int device, sl_handle **out) {
     if (!out) return SL_E_ARG;
     *out = nullptr;
@@ -609,6 +620,8 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
         return fail(h, SL_E_ARG, "scene array pointer is NULL");
     if (P == 0 && C != 0) return fail(h, SL_E_ARG, "curves without paths");
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    int src = settle(h);
+    if (src) return src;
     // The arrays are packed into one pinned staging block (no page faults after the first
     // scene of a size, DMA without the runtime's bounce copies), then copied asynchronously.
     auto al = [](size_t x) { return (x + 63) & ~(size_t)63; };
@@ -802,14 +815,25 @@ int sl_rasterize(sl_handle *h, void *stream) {
     // (every kernel runs again; only its counts and sizes are taken over), queued on the
     // stream, and the call returns at once.  Step timing and parity dumps need the host
     // reads of a full frame, so they keep every frame synchronous.
+    // A frame still queued on another stream shares this frame's buffers: a replay waits
+    // for it on the device, a full frame (which may reallocate) on the host.
     const hipStream_t s = (hipStream_t)stream;
     const bool replay = h->sized && !h->timing && !h->debug;
+    if (h->pending && (!replay || s != h->pending_stream)) {
+        if (replay) {
+            HIPCHK(h, hipStreamWaitEvent(s, h->done_ev, 0));
+        } else {
+            const int src = settle(h);
+            if (src) return src;
+        }
+    }
     const int rc = render(h, s, replay);
     h->pending = false;
     if (rc) return rc;
     if (replay) {
         HIPCHK(h, hipEventRecord(h->done_ev, s));
         h->pending = true;
+        h->pending_stream = s;
     }
     return SL_OK;
 }
@@ -825,10 +849,8 @@ int sl_output_device_ptr(sl_handle *h, void **dev_ptr, int *pitch_bytes) {
 int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags) {
     if (!h || !host_rgba) return SL_E_ARG;
     if (!h->b_image.p) return fail(h, SL_E_STATE, "no image rendered yet");
-    if (h->pending) {
-        HIPCHK(h, hipEventSynchronize(h->done_ev));
-        h->pending = false;
-    }
+    int rc = settle(h);
+    if (rc) return rc;
     const size_t row = (size_t)h->W * 4;
     HIPCHK(h, hipMemcpy(host_rgba, h->b_image.p, row * h->H, hipMemcpyDeviceToHost));
     if (flags & 1) flip_rows_alpha(host_rgba, h->W, h->H);
@@ -921,6 +943,8 @@ int sl_debug_requantize_sweep(sl_handle *h, int channel, uint32_t bits_lo, uint3
 
 int sl_debug_dump(sl_handle *h, const char *name, void *host_dst, size_t capacity, size_t *nbytes) {
     if (!h || !name || !nbytes) return SL_E_ARG;
+    int rc = settle(h);
+    if (rc) return rc;
     const void *src = nullptr;
     size_t n = 0;
     std::string k(name);

@@ -134,7 +134,10 @@ int sl_clear(sl_handle *h);
  * done.  Later frames of the unchanged scene reuse those counts and sizes: every kernel
  * runs again, queued on `stream` with no host synchronisation, and the call returns at
  * once (step timing and debug dumps keep every frame synchronous).  sl_read_output waits
- * for such a frame; users of sl_output_device_ptr order their work after it on `stream`. */
+ * for such a frame; users of sl_output_device_ptr order their work after it on `stream`.
+ * A queued frame shares the handle's buffers with the next one: sl_add_vg, sl_debug_dump
+ * and a full frame wait for it on the host, and a queued frame on another stream waits for
+ * it on the device. */
 int sl_rasterize(sl_handle *h, void *stream);
 
 /* Device pointer and row pitch (bytes) of the handle's RGBA8 output image. */

@@ -443,3 +443,37 @@ def test_rerender_on_a_torch_stream(rasterizer):
     st.synchronize()
     assert np.abs(img.astype(np.int32) - ref.astype(np.int32)).max() <= 1
     np.testing.assert_array_equal(dev.cpu().numpy(), img)
+
+
+def test_queued_frames_then_new_scene_and_second_stream(rasterizer):
+    """Frames still queued on a caller's non-blocking stream share the handle's buffers:
+    uploading the next scene waits for them, and a frame issued on a second stream is
+    ordered after them, so every read-back is the scene last added."""
+    import torch
+    W, H, spp = 512, 384, 16
+    a = scanline.generate(scanline.scene.KIND_MIXED, W, H, 400, 0x5EED0401)
+    b = scanline.generate(scanline.scene.KIND_CUBIC_CHAINS, W, H, 60, 0x5EED0402)
+    r = rasterizer
+    r.setSamples(spp)
+    r.enableSRGBCorrection(True)
+    r.setMultisampleOutput(False)
+    r.resizeOutputBuffer(W, H)
+    r.setInputTransform(np.eye(4, dtype=np.float32))
+    r.dumpDebugData(False)
+    r.enableStepTiming(False)
+    ref_b = pyoracle.run(b, W, H, spp, srgb=True).get("image").reshape(H, W, 4)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    r.addVg(a)
+    r.rasterize(s1.cuda_stream)            # full frame: sizes the buffers
+    for _ in range(6):
+        r.rasterize(s1.cuda_stream)        # queued, not waited for
+    r.addVg(b)                             # overwrites the scene buffers
+    r.rasterize(s1.cuda_stream)
+    img = r.output()
+    assert np.abs(img.astype(np.int32) - ref_b.astype(np.int32)).max() <= 1
+    for _ in range(4):
+        r.rasterize(s1.cuda_stream)
+    r.rasterize(s2.cuda_stream)            # same buffers, another stream
+    img2 = r.output()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(img2, img)
