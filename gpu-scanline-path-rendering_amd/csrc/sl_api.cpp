@@ -997,6 +997,10 @@ struct sl_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int shape[4] = {0, 0, 4, 256};        // resolve grid, class grid, compositor group, fragment block
+    // the last frame issued by sl_batch_render(_eager), possibly still running
+    hipEvent_t done_ev = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool pending = false;
     std::string err;
 };
 
@@ -1010,6 +1014,27 @@ int bfail(sl_batch *b, int code, const char *fmt, ...) {
     va_end(ap);
     if (b) b->err = buf;
     return code;
+}
+
+// Every frame of a batch uses the same buffers.  A frame on another stream than the last
+// one waits for it on the device; tuning and recapture wait on the host.
+int batch_order(sl_batch *b, hipStream_t s) {
+    if (b->pending && s != b->last_stream && hipStreamWaitEvent(s, b->done_ev, 0) != hipSuccess)
+        return bfail(b, SL_E_HIP, "hipStreamWaitEvent failed");
+    return SL_OK;
+}
+
+int batch_issued(sl_batch *b, hipStream_t s) {
+    if (hipEventRecord(b->done_ev, s) != hipSuccess) return bfail(b, SL_E_HIP, "hipEventRecord failed");
+    b->pending = true;
+    b->last_stream = s;
+    return SL_OK;
+}
+
+int batch_settle(sl_batch *b) {
+    if (b->pending && hipEventSynchronize(b->done_ev) != hipSuccess) return bfail(b, SL_E_HIP, "render failed");
+    b->pending = false;
+    return SL_OK;
 }
 
 void drop_graph(sl_batch *b) {
@@ -1153,6 +1178,7 @@ int sl_batch_create(int device, const sl_scene *scenes, const int *widths, const
             (rc = ensure(h, h->scarry, ids.size() * 4))) { b->err = h->err; break; }
     }
     if (!rc && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) rc = SL_E_HIP;
+    if (!rc && hipEventCreateWithFlags(&b->done_ev, hipEventDisableTiming) != hipSuccess) rc = SL_E_HIP;
     if (!rc) {
         // one eager frame sizes every buffer; the graph then replays it without host syncs
         apply_shape(b);
@@ -1170,9 +1196,12 @@ int sl_batch_create(int device, const sl_scene *scenes, const int *widths, const
 
 int sl_batch_destroy(sl_batch *b) {
     if (!b) return SL_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
     drop_graph(b);
     for (sl_handle *h : b->grp) sl_destroy(h);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    if (b->done_ev) (void)hipEventDestroy(b->done_ev);
     delete b;
     return SL_OK;
 }
@@ -1180,15 +1209,20 @@ int sl_batch_destroy(sl_batch *b) {
 int sl_batch_render(sl_batch *b, void *stream) {
     if (!b || !b->exec) return SL_E_ARG;
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, SL_E_HIP, "hipSetDevice failed");
+    int rc = batch_order(b, (hipStream_t)stream);
+    if (rc) return rc;
     const hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
     if (e != hipSuccess) return bfail(b, SL_E_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
-    return SL_OK;
+    return batch_issued(b, (hipStream_t)stream);
 }
 
 int sl_batch_render_eager(sl_batch *b, void *stream) {
     if (!b) return SL_E_ARG;
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, SL_E_HIP, "hipSetDevice failed");
-    return enqueue_all(b, (hipStream_t)stream, true);
+    int rc = batch_order(b, (hipStream_t)stream);
+    if (!rc) rc = enqueue_all(b, (hipStream_t)stream, true);
+    if (!rc) rc = batch_issued(b, (hipStream_t)stream);
+    return rc;
 }
 
 int sl_batch_set_shape(sl_batch *b, const int shape[4]) {
@@ -1196,6 +1230,8 @@ int sl_batch_set_shape(sl_batch *b, const int shape[4]) {
     if (shape[0] < 0 || shape[1] < 0 || (shape[2] != 4 && shape[2] != 8) ||
         (shape[3] != 64 && shape[3] != 128 && shape[3] != 256))
         return bfail(b, SL_E_ARG, "launch shape out of range");
+    int rc = batch_settle(b);
+    if (rc) return rc;
     for (int i = 0; i < 4; ++i) b->shape[i] = shape[i];
     apply_shape(b);
     return capture(b);
@@ -1211,6 +1247,7 @@ int sl_batch_tune(sl_batch *b, int reps, double *best_ms) {
     if (!b) return SL_E_ARG;
     if (reps <= 0) reps = 3;
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, SL_E_HIP, "hipSetDevice failed");
+    if (int rc0 = batch_settle(b)) return rc0;
     // candidate launch shapes: per-bucket resolve grid, size-class grid, compositor
     // bucket group, fragment block size
     const int rg[] = {4096, 16384, 65536};
@@ -1276,6 +1313,7 @@ int sl_batch_read_output(sl_batch *b, int scene, uint8_t *host_rgba, int flags) 
     if (!b || scene < 0 || scene >= b->n || !host_rgba) return SL_E_ARG;
     if (hipSetDevice(b->device) != hipSuccess) return bfail(b, SL_E_HIP, "hipSetDevice failed");
     if (hipDeviceSynchronize() != hipSuccess) return bfail(b, SL_E_HIP, "render failed");
+    b->pending = false;
     sl_handle *h = b->grp[b->scene_grp[scene]];
     const int W = b->scene_w[scene], H = b->scene_h[scene];
     const size_t bytes = (size_t)W * H * 4;

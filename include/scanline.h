@@ -202,7 +202,9 @@ int sl_debug_requantize_sweep(sl_handle *h, int channel, uint32_t bits_lo, uint3
  * `stream`; sl_batch_render_eager issues the same kernels without the graph.
  * sl_batch_tune times candidate launch shapes {per-bucket resolve grid, size-class
  * grid, compositor bucket group, fragment block size} on this batch, keeps the fastest
- * and recaptures.  Images are read per scene. */
+ * and recaptures.  Images are read per scene.  A frame issued on another stream than the
+ * previous one waits for it on the device; tuning, a shape change and the read-back wait
+ * for it on the host. */
 typedef struct sl_batch sl_batch;
 int sl_batch_create(int device, const sl_scene *scenes, const int *widths, const int *heights,
                     const int *samples, int n_scenes, int srgb, sl_batch **out);

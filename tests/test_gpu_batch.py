@@ -115,3 +115,32 @@ def test_bad_batch_arguments():
         scanline.BatchRasterizer([s], [(32, 32)], [12])
     with pytest.raises(scanline.ScanlineError):
         scanline.BatchRasterizer([s], [(0, 32)], [8])
+
+
+def test_frames_on_two_streams_and_retuning(small_batch):
+    """Replays and eager frames alternate between two non-blocking streams without host
+    waits, and a shape change and a tuning run follow queued frames: each frame is
+    ordered after the last, so every image stays the single-scene image."""
+    import torch
+    scenes, sizes, spp = small_batch
+    scenes, sizes, spp = scenes[:16], sizes[:16], spp[:16]
+    ref = single_images(scenes, sizes, spp)
+    b = scanline.BatchRasterizer(scenes, sizes, spp)
+    try:
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        for k in range(4):
+            b.render(s1.cuda_stream)
+            b.render_eager(s2.cuda_stream)
+            b.render(s2.cuda_stream if k % 2 else s1.cuda_stream)
+        for i in range(len(scenes)):
+            np.testing.assert_array_equal(b.output(i), ref[i], err_msg="scene %d two streams" % i)
+        b.render(s1.cuda_stream)
+        b.set_shape((16384, 4096, 8, 128))
+        b.render(s2.cuda_stream)
+        b.tune(reps=1)
+        b.render(s1.cuda_stream)
+        for i in range(len(scenes)):
+            np.testing.assert_array_equal(b.output(i), ref[i], err_msg="scene %d after retuning" % i)
+        torch.cuda.synchronize()
+    finally:
+        b.close()
