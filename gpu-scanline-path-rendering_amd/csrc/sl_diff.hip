@@ -428,6 +428,10 @@ struct sl_diff {
     uint32_t NX = 0;
     bool have_pre = false;
     const float *last_points = nullptr, *last_colors = nullptr;
+    // the last forward or backward, possibly still running on last_stream
+    hipEvent_t done_ev = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool pending = false;
     std::string err;
 };
 
@@ -441,6 +445,25 @@ int dfail(sl_diff *d, int code, const char *fmt, ...) {
     va_end(ap);
     if (d) d->err = buf;
     return code;
+}
+
+// Forward and backward share the handle's buffers.  Work issued on another stream than
+// the last call's waits on the host until that call has finished.
+int diff_order(sl_diff *d, hipStream_t s) {
+    if (d->pending && s != d->last_stream) {
+        if (hipEventSynchronize(d->done_ev) != hipSuccess) return dfail(d, SL_E_HIP, "previous call failed");
+        d->pending = false;
+    }
+    return SL_OK;
+}
+
+int diff_issued(sl_diff *d, hipStream_t s) {
+    if (!d->done_ev && hipEventCreateWithFlags(&d->done_ev, hipEventDisableTiming) != hipSuccess)
+        return dfail(d, SL_E_HIP, "hipEventCreate failed");
+    if (hipEventRecord(d->done_ev, s) != hipSuccess) return dfail(d, SL_E_HIP, "hipEventRecord failed");
+    d->pending = true;
+    d->last_stream = s;
+    return SL_OK;
 }
 
 int dens(sl_diff *d, Buf &b, size_t bytes) {
@@ -525,6 +548,7 @@ int sl_diff_destroy(sl_diff *d) {
                  &d->boff, &d->bcur, &d->order, &d->sx, &d->sd, &d->df, &d->cov, &d->pre, &d->gcov, &d->scan_tmp};
     for (Buf *b : bs)
         if (b->p) (void)hipFree(b->p);
+    if (d->done_ev) (void)hipEventDestroy(d->done_ev);
     delete d;
     return SL_OK;
 }
@@ -536,6 +560,7 @@ int sl_diff_forward(sl_diff *d, const float *points, const float *colors, float 
     if (!d || !points || !colors || !image) return SL_E_ARG;
     if (hipSetDevice(d->device) != hipSuccess) return dfail(d, SL_E_HIP, "hipSetDevice failed");
     hipStream_t s = (hipStream_t)stream;
+    if (int rc0 = diff_order(d, s)) return rc0;
     const DiffDev D = dev_view(d);
     const float2 *pts = (const float2 *)points;
     const uint32_t C = d->C;
@@ -587,7 +612,7 @@ int sl_diff_forward(sl_diff *d, const float *points, const float *colors, float 
     d->have_pre = keep_for_backward != 0;
     d->last_points = points;
     d->last_colors = colors;
-    return SL_OK;
+    return diff_issued(d, s);
 }
 
 int sl_diff_backward(sl_diff *d, const float *grad_image, float *grad_points, float *grad_colors, void *stream) {
@@ -595,6 +620,7 @@ int sl_diff_backward(sl_diff *d, const float *grad_image, float *grad_points, fl
     if (!d->have_pre) return dfail(d, SL_E_STATE, "backward needs a forward with keep_for_backward");
     if (hipSetDevice(d->device) != hipSuccess) return dfail(d, SL_E_HIP, "hipSetDevice failed");
     hipStream_t s = (hipStream_t)stream;
+    if (int rc0 = diff_order(d, s)) return rc0;
     const DiffDev D = dev_view(d);
     const size_t NP = (size_t)d->W * d->H;
     int rc;
@@ -611,7 +637,7 @@ int sl_diff_backward(sl_diff *d, const float *grad_image, float *grad_points, fl
                            (const float *)BP<float>(d->gcov), grad_points);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return dfail(d, SL_E_HIP, "backward launch failed: %s", hipGetErrorString(e));
-    return SL_OK;
+    return diff_issued(d, s);
 }
 
 int sl_diff_crossings(sl_diff *d, int64_t *n) {

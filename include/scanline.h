@@ -230,7 +230,8 @@ const char *sl_batch_last_error(sl_batch *b);
  * points and colours must stay unchanged until the backward).  sl_diff_backward writes
  * d loss / d points and d loss / d colors given d loss / d image: the compositing chain
  * reversed per pixel, and boundary terms at every sample-row crossing with a fill change.
- * One pending backward per handle. */
+ * One pending backward per handle.  A call on another stream than the previous call waits
+ * on the host for that call to finish (the calls share the handle's buffers). */
 typedef struct sl_diff sl_diff;
 int sl_diff_create(int device, int width, int height, int sample_rows, uint32_t n_paths,
                    uint32_t n_curves, uint32_t n_vertices, const uint32_t *curve_vertex_pos,

@@ -162,3 +162,36 @@ def test_adam_recovers_the_target(setup):
     # are not identifiable from the image, so the test does not ask for the points)
     assert losses[-1] < 0.01 * losses[0]
     assert c1 < 0.7 * c0
+
+
+def test_forward_and_backward_on_alternating_streams(setup):
+    """The handle's kept buffers are shared by every call: a forward on one stream and
+    the backward on another, then the reverse, give what one stream gives."""
+    import torch
+    scene, W, H, r, uniq, index, tgt, _, _ = setup
+    pts, col = r.points0.clone(), r.colors0.clone()
+    col2 = col.clone()
+    col2[:, :3] = col2[:, :3].flip(0)
+    g = (2 * (r.forward_raw(pts, col, keep=False) - tgt)).contiguous()
+    torch.cuda.synchronize()
+    ref = []
+    for c in (col, col2):
+        img = r.forward_raw(pts, c)
+        gp, gc = r.backward_raw(g)
+        torch.cuda.synchronize()
+        ref.append((img.cpu(), gp.cpu(), gc.cpu()))
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    got = []
+    for c, (sa, sb) in zip((col, col2), ((s1, s2), (s2, s1))):
+        sa.wait_stream(torch.cuda.current_stream())
+        sb.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(sa):
+            img = r.forward_raw(pts, c)
+        with torch.cuda.stream(sb):
+            gp, gc = r.backward_raw(g)
+        torch.cuda.synchronize()
+        got.append((img.cpu(), gp.cpu(), gc.cpu()))
+    for (ri, rp, rc), (gi, gp, gc) in zip(ref, got):
+        assert torch.equal(gi, ri)
+        assert torch.allclose(gp, rp, rtol=1e-5, atol=1e-6 * float(rp.abs().max()))
+        assert torch.allclose(gc, rc, rtol=1e-5, atol=1e-6 * float(rc.abs().max()))
