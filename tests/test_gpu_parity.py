@@ -477,3 +477,27 @@ def test_queued_frames_then_new_scene_and_second_stream(rasterizer):
     img2 = r.output()
     torch.cuda.synchronize()
     np.testing.assert_array_equal(img2, img)
+
+
+def test_image_dump_after_queued_frames(rasterizer):
+    """A parity dump read straight after frames queued on a caller's stream sees the
+    finished image: the dump waits for the queued frames."""
+    import torch
+    W, H, spp = 256, 224, 8
+    scene = scanline.generate(scanline.scene.KIND_MIXED, W, H, 150, 0x5EED0501)
+    r = rasterizer
+    r.setSamples(spp)
+    r.enableSRGBCorrection(True)
+    r.setMultisampleOutput(False)
+    r.resizeOutputBuffer(W, H)
+    r.setInputTransform(np.eye(4, dtype=np.float32))
+    r.dumpDebugData(False)
+    r.enableStepTiming(False)
+    r.addVg(scene)
+    ref = pyoracle.run(scene, W, H, spp, srgb=True).get("image").reshape(H, W, 4)
+    st = torch.cuda.Stream()
+    for _ in range(6):
+        r.rasterize(st.cuda_stream)
+    img = r.dump("image").reshape(H, W, 4)
+    assert np.abs(img.astype(np.int32) - ref.astype(np.int32)).max() <= 1
+    torch.cuda.synchronize()
