@@ -32,7 +32,9 @@ _libs = {}
 
 
 def lib(omp=False):
-    path = LIB_OMP if omp else LIB_SERIAL
+    # SCANLINE_ORACLE_LIB names another build of the serial oracle (the mutation tests
+    # point it at copies with a planted mistake)
+    path = LIB_OMP if omp else (os.environ.get("SCANLINE_ORACLE_LIB") or LIB_SERIAL)
     if path in _libs:
         return _libs[path]
     if not os.path.exists(path):

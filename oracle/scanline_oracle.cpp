@@ -17,6 +17,11 @@
  *     clamp(floor(u*256), 0, 255);
  *   - the OpenGL compositor is restated as src-over blending into an 8-bit
  *     framebuffer that is re-quantised after every blend.
+ * Pinning (tests/, CPU): orientation, transform, nudge, culling flags, monotone split and
+ * curved intersections by test_oracle_geometry_pins.py, whose failure on planted
+ * mistakes is checked by test_oracle_mutations.py; sample sets, tables, masks, solvers,
+ * scans, stencil and compositing by test_oracle_pinning.py, including the reference's own
+ * table tolerance test (ras_qm_mask.h:600-714).
  * Citations are file:line into the reference tree.
  */
 #include "scanline_oracle.h"
