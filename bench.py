@@ -23,28 +23,39 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-scanline-path-rendering_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0
+# host cores this process may use, read before an OpenMP runtime binds the main thread
+try:
+    NPROC = len(os.sched_getaffinity(0))
+except AttributeError:
+    NPROC = os.cpu_count()
+# the CPU baseline's threads stay on their cores (read when the OpenMP runtime loads)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 
 def stage_bytes(stage, cnt, spp, W, H, ms=False):
-    """Algorithmic HBM bytes of one stage's launch (each datum read or written once).  The
-    compositor's per-bucket row records (24 B per non-empty bucket, under 10 % of its bytes)
-    are not counted."""
-    m = spp // 2  # mask bytes per 2x2 cell
+    """Algorithmic HBM bytes of one stage's launch: the whole-frame figure
+    B_alg = 8V + 14C + 5P + F*2*(8 + m + 9) + M*2m + S*2*8 + 4WH (each datum written once
+    and read once; m = spp/2 bytes of mask per 2x2 cell) split by the stage that writes or
+    reads each term:
+      intersect  the scene's vertices and curves (8V + 14C) in, intersections (8F) out;
+      fragments  intersections (8F) in, mask + key + path|sign + winding ((m+9)F) out;
+      resolve    those ((m+9)F) in, resolved masks (mM) and spans (8S) out;
+      composite  resolved masks (mM) and spans (8S) in, the RGBA8 image (4WH) out.
+    The renderer's own encodings are smaller than these (4-byte items, per-pixel sample
+    counts), so the bytes it moves are fewer than counted here."""
+    m = spp // 2
     F, M, S = cnt["curve_fragment"], cnt["merged_fragment"], cnt["span"]
     C, V, P = cnt["curve"], cnt["vertex"], cnt["path"]
-    items = M + S
-    if stage == "intersect":      # curve (type, vpos, path, cuts, count, offset) + vertices, ts out
-        return 30 * C + 8 * V + 8 * F
-    if stage == "fragments":      # ts in; per fragment record out (meta 4 + mask m)
-        return 8 * F + (4 + m) * F
-    if stage == "resolve":        # records in; items (4 B) + per-pixel counts (coverage) or masks out
-        return (4 + m) * F + 4 * items + (m if ms else 4) * M
-    if stage == "composite":      # items + per-pixel sample counts (coverage) or masks in; RGBA8 out
-        return 4 * items + (m if ms else 4) * M + 4 * W * H
+    if stage == "intersect":
+        return 8 * V + 14 * C + 8 * F
+    if stage == "fragments":
+        return 8 * F + (m + 9) * F
+    if stage == "resolve":
+        return (m + 9) * F + m * M + 8 * S
+    if stage == "composite":
+        return m * M + 8 * S + 4 * W * H
     if stage == "transform_cull":
-        return 8 * V + 4 * V + 8 * V + 4 * P
-    if stage == "monotone_count":
-        return 30 * C + 8 * V + 21 * C
+        return 8 * V + 5 * P
     return 0
 
 
@@ -90,26 +101,33 @@ def b_alg_total(cnt, spp, W, H):
             + M * 2 * m + S * 2 * 8 + W * H * 4)
 
 
-def cpu_baseline(scene_full, W, H, spp, sample_paths, seed, kind):
-    """OpenMP oracle on the first `sample_paths` paths of the same scene generator."""
-    import pyoracle
-    import scanline
+def cpu_baseline(scene, W, H, spp, ms, warmup, reps):
+    """The OpenMP build of the oracle on the same scene: wall time of its render (the
+    scene already in memory, tables built; transform through the RGBA8 image), `warmup`
+    untimed renders, then the median of `reps`, on every host core this process may use."""
+    import statistics
 
-    sub = scanline.generate(kind, W, H, sample_paths, seed)
+    import pyoracle
+
     L = pyoracle.lib(omp=True)
     threads = L.oracle_threads()
     times, frags = [], 0
-    for _ in range(3):
-        r = pyoracle.run(sub, W, H, spp, srgb=True, ms=False, keep=False, omp=True)
-        times.append(r.seconds)
+    t_all = time.perf_counter()
+    for k in range(warmup + reps):
+        r = pyoracle.run(scene, W, H, spp, srgb=True, ms=ms, keep=False, omp=True)
+        if k >= warmup:
+            times.append(r.seconds)
         frags = r.counts()["curve_fragment"]
         del r
-    t = sorted(times)[len(times) // 2]
+    t = statistics.median(times)
     return {"value": round(frags / t / 1e6, 3), "unit": "Mfragments/s", "cores": threads,
             "kind": "port",
-            "sample": "%d of %d paths of the same seeded scene at %dx%d, %d spp, coverage output; "
-                      "median of 3 renders (%.2f s each, F=%d)" % (sample_paths, scene_full.n_paths, W, H,
-                                                                  spp, t, frags)}
+            "sample": "the full scene (%d paths at %dx%d, %d spp, %s output, F=%d): median of %d renders "
+                      "after %d warm-ups, %.3f s each (min %.3f, max %.3f); %.0f s in all"
+                      % (scene.n_paths, W, H, spp, "multisample" if ms else "coverage", frags, reps, warmup, t,
+                         min(times), max(times), time.perf_counter() - t_all),
+            "nproc": NPROC, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "render_s_median": round(t, 4)}
 
 
 def bench_batch(args):
@@ -280,7 +298,8 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--ms", action="store_true", help="multisample (per-sample) compositing")
-    ap.add_argument("--cpu-sample-paths", type=int, default=2000)
+    ap.add_argument("--cpu-warmup", type=int, default=2, help="untimed CPU-baseline renders")
+    ap.add_argument("--cpu-reps", type=int, default=10, help="timed CPU-baseline renders (median)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
@@ -314,12 +333,10 @@ def main():
         W = H = 4096
         args.paths, args.spp = 100000, 16
         workload = "text-like scene, 4096x4096 canvas, 100k glyphs, 16 spp"
-        cpu_paths = args.cpu_sample_paths * 5
     else:
         kind, seed = scanline.scene.KIND_MIXED, scanline.scene.SEED_BASE ^ 2
         W = H = args.size
         workload = "plain renderer, %dx%d canvas, %d mixed paths, %d spp" % (W, H, args.paths, args.spp)
-        cpu_paths = args.cpu_sample_paths
     scene = scanline.generate(kind, W, H, args.paths, seed)
     r = scanline.VGRasterizer(0)
     r.setSamples(args.spp)
@@ -391,7 +408,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, W, H, args.spp, cpu_paths, seed, kind)
+        cpu = cpu_baseline(scene, W, H, args.spp, args.ms, args.cpu_warmup, args.cpu_reps)
 
     out = {
         "metric": "Mfragments/s + Mpixels/s at 2048^2 32spp (GB/s vs HBM peak); bwd/fwd time",

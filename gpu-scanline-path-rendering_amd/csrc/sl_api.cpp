@@ -29,9 +29,20 @@ struct DevBuf {
     size_t cap = 0;
 };
 
-struct Stage {
-    const char *name;
+// The per-frame device arrays of one frame.  A handle keeps one arena that grows by
+// hipMalloc and is reused frame to frame; the allocation-free frame builds a temporary
+// arena whose arrays are carved out of the library's static scratch instead.
+struct Arena {
+    DevBuf b_pcnt, b_nchain, b_choff, b_chains, b_chain_t;
+    DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
+        b_boff, b_bwsum, b_bcarry, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
+        b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
+        b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_image, b_scan_tmp, b_lcount;
+    // carving state: non-null base = every array comes from [base, base + cap)
+    uint8_t *carve_base = nullptr;
+    size_t carve_cap = 0, carve_off = 0;
 };
+
 const char *kStageNames[] = {"transform_cull", "monotone_count", "intersect", "bucket_setup",
                              "fragments",      "resolve",        "order_rows", "composite"};
 constexpr int kStages = 8;
@@ -74,11 +85,7 @@ struct sl_handle {
     DevBuf ptable[3];
     DevBuf ctab[2];  // [srgb] : dec_c, thr_c, dec_a, thr_a, lut_c, lut_a packed
     // per-frame arena
-    DevBuf b_pcnt, b_nchain, b_choff, b_chains, b_chain_t;
-    DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
-        b_boff, b_bwsum, b_bcarry, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
-        b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
-        b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_image, b_scan_tmp, b_lcount;
+    Arena A;
     // debug host copies
     std::vector<int32_t> d_ts, d_yx, d_pid, d_draw;
     std::vector<int8_t> d_wn;
@@ -91,6 +98,11 @@ struct sl_handle {
     hipEvent_t done_ev = nullptr;
     bool pending = false;
     hipStream_t pending_stream = nullptr;  // the stream done_ev was recorded on
+    // recorded on pending_stream when a frame moves to another stream: everything queued
+    // there so far (the frame and the caller's reads of its image) precedes the new frame
+    hipEvent_t switch_ev = nullptr;
+    // size of the image the last frame wrote (the output size may change since)
+    int img_W = 0, img_H = 0;
     int64_t NB = 0;
     uint32_t *nz_vals_sorted = nullptr;
 };
@@ -137,6 +149,24 @@ void release(DevBuf &b) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+}
+
+// ensure() for a frame's arena: a carved arena hands out 256-byte aligned pieces of its
+// block (a piece that is too small is replaced by a new one; nothing is ever freed).
+int ensure_in(sl_handle *h, Arena &A, DevBuf &b, size_t bytes) {
+    if (!A.carve_base) return ensure(h, b, bytes);
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return SL_OK;
+    const size_t at = (A.carve_off + 255) & ~(size_t)255;
+    if (at > A.carve_cap || bytes > A.carve_cap - at)
+        return fail(h, SL_E_UNSUPPORTED,
+                    "the allocation-free frame needs more than its %zu MiB of static scratch for this scene "
+                    "(%zu bytes in use, %zu more asked); render it with a handle's own arena",
+                    A.carve_cap >> 20, at, bytes);
+    b.p = A.carve_base + at;
+    b.cap = bytes;
+    A.carve_off = at + bytes;
+    return SL_OK;
 }
 
 template <class T> T *P_(DevBuf &b) { return (T *)b.p; }
@@ -207,6 +237,10 @@ int ensure_tables(sl_handle *h) {
     return SL_OK;
 }
 
+bool tables_ready(sl_handle *h) {
+    return h->ptable[spp_slot(h->spp)].p && h->ctab[h->srgb ? 1 : 0].p;
+}
+
 sl::Tables tables_view(sl_handle *h) {
     sl::Tables t;
     t.spp = h->spp;
@@ -248,12 +282,22 @@ uint32_t read_u32(sl_handle *h, const uint32_t *dptr, hipStream_t s, int &rc) {
 // One frame.  `replay` re-issues the work of the previous eager render of the same scene
 // without any host synchronisation (buffer sizes and counts are taken from that render),
 // so that the launch sequence can be captured into a HIP graph or timed back to back.
-int render(sl_handle *h, hipStream_t s, bool replay = false) {
+// A frame's arrays come from `A`: the handle's own arena, or a carved one (the
+// allocation-free frame), which must then name the caller's image `out_image` / `out_pitch`.
+int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_image = nullptr, int out_pitch = 0) {
     int rc;
+    const bool carved = A.carve_base != nullptr;
     if (replay && !h->sized) return fail(h, SL_E_STATE, "replay before an eager render of the scene");
     if (!h->has_scene) return fail(h, SL_E_STATE, "rasterize called before add_vg");
     if (h->W <= 0 || h->H <= 0) return fail(h, SL_E_STATE, "output size not set (resize_output_buffer)");
-    if ((rc = ensure_tables(h))) return rc;
+    if (carved) {
+        if (replay || h->debug || h->nscenes || !out_image) return fail(h, SL_E_STATE, "carved frame misuse");
+        if (!tables_ready(h))
+            return fail(h, SL_E_STATE, "mask and colour tables for %d spp are not built (they are built by "
+                        "sl_add_vg, sl_set_samples and sl_enable_srgb_correction)", h->spp);
+    } else if ((rc = ensure_tables(h))) {
+        return rc;
+    }
     const sl::SceneDev sc = scene_view(h);
     const sl::Tables tb = tables_view(h);
     const int NW = tb.words;
@@ -265,7 +309,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         for (int j = 0; j < 4; ++j) f.m[4 * i + j] = h->m[4 * j + i];
     f.srgb = h->srgb; f.ms = h->ms;
     f.keep_masks = (h->ms || h->debug) ? 1 : 0;
-    f.pitch = h->W * 4;
+    f.pitch = carved ? out_pitch : h->W * 4;
     f.R = h->nscenes ? h->batch_rows : (int64_t)(h->H + 1) / 2;
     f.nscenes = h->nscenes;
     f.ntiles = h->batch_tiles;
@@ -280,24 +324,26 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     size_t scan_need = sl::scan_tmp_elems_needed((size_t)std::max(C, P) + 1);
     int rc2 = SL_OK;
     (void)rc2;
-    if ((rc = ensure(h, h->b_tvtx, (size_t)V * 8)) || (rc = ensure(h, h->b_pflags, (size_t)P * 4)) ||
-        (rc = ensure(h, h->b_prow, (size_t)P * 8)) || (rc = ensure(h, h->b_ccuts, (size_t)C * 16)) ||
-        (rc = ensure(h, h->b_cncut, (size_t)C)) || (rc = ensure(h, h->b_cfrag, ((size_t)C + 1) * 4)) ||
-        (rc = ensure(h, h->b_coff, ((size_t)C + 1) * 4)) || (rc = ensure(h, h->b_pnb, ((size_t)P + 1) * 4)) ||
-        (rc = ensure(h, h->b_pcnt, (size_t)C * 20)) || (rc = ensure(h, h->b_nchain, ((size_t)C + 1) * 4)) ||
-        (rc = ensure(h, h->b_choff, ((size_t)C + 1) * 4)) ||
-        (rc = ensure(h, h->b_pbase, ((size_t)P + 1) * 4)) || (rc = ensure(h, h->b_small, 64)) ||
-        (rc = ensure(h, h->b_lcount, 1024)) ||
-        (rc = ensure(h, h->b_image, image_bytes)) || (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
+    if ((rc = ensure_in(h, A, A.b_tvtx, (size_t)V * 8)) || (rc = ensure_in(h, A, A.b_pflags, (size_t)P * 4)) ||
+        (rc = ensure_in(h, A, A.b_prow, (size_t)P * 8)) || (rc = ensure_in(h, A, A.b_ccuts, (size_t)C * 16)) ||
+        (rc = ensure_in(h, A, A.b_cncut, (size_t)C)) || (rc = ensure_in(h, A, A.b_cfrag, ((size_t)C + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_coff, ((size_t)C + 1) * 4)) || (rc = ensure_in(h, A, A.b_pnb, ((size_t)P + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_pcnt, (size_t)C * 20)) || (rc = ensure_in(h, A, A.b_nchain, ((size_t)C + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_choff, ((size_t)C + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_pbase, ((size_t)P + 1) * 4)) || (rc = ensure_in(h, A, A.b_small, 64)) ||
+        (rc = ensure_in(h, A, A.b_lcount, 1024)) ||
+        (!carved && (rc = ensure_in(h, A, A.b_image, image_bytes))) ||
+        (rc = ensure_in(h, A, A.b_scan_tmp, scan_need * 4)))
         return rc;
-    f.tvtx = P_<float2>(h->b_tvtx); f.pflags = P_<uint32_t>(h->b_pflags); f.prow = P_<int2>(h->b_prow);
-    f.ccuts = P_<float4>(h->b_ccuts); f.cncut = P_<uint8_t>(h->b_cncut); f.cfrag = P_<uint32_t>(h->b_cfrag);
-    f.coff = P_<uint32_t>(h->b_coff); f.pnb = P_<uint32_t>(h->b_pnb); f.pbase = P_<uint32_t>(h->b_pbase);
-    f.pcnt = P_<uint32_t>(h->b_pcnt); f.nchain = P_<uint32_t>(h->b_nchain); f.choff = P_<uint32_t>(h->b_choff);
-    f.image = P_<uint8_t>(h->b_image);
-    f.big_count = P_<uint32_t>(h->b_small);
-    f.err_flag = P_<uint32_t>(h->b_small) + 1;
-    f.scan_tmp = P_<uint32_t>(h->b_scan_tmp);
+    f.tvtx = P_<float2>(A.b_tvtx); f.pflags = P_<uint32_t>(A.b_pflags); f.prow = P_<int2>(A.b_prow);
+    f.ccuts = P_<float4>(A.b_ccuts); f.cncut = P_<uint8_t>(A.b_cncut); f.cfrag = P_<uint32_t>(A.b_cfrag);
+    f.coff = P_<uint32_t>(A.b_coff); f.pnb = P_<uint32_t>(A.b_pnb); f.pbase = P_<uint32_t>(A.b_pbase);
+    f.pcnt = P_<uint32_t>(A.b_pcnt); f.nchain = P_<uint32_t>(A.b_nchain); f.choff = P_<uint32_t>(A.b_choff);
+    f.image = carved ? out_image : P_<uint8_t>(A.b_image);
+    if (!carved) { h->img_W = h->W; h->img_H = h->H; }
+    f.big_count = P_<uint32_t>(A.b_small);
+    f.err_flag = P_<uint32_t>(A.b_small) + 1;
+    f.scan_tmp = P_<uint32_t>(A.b_scan_tmp);
 
     auto mark = [&](int i) {
         if (h->timing && !replay) (void)hipEventRecord(h->ev[i], s);
@@ -317,7 +363,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
         F = h->last_F;
         NCH = h->last_NCH;
     } else {
-        h->sized = false;
+        if (!carved) h->sized = false;
         HIPCHK(h, hipMemcpyAsync(&F, f.coff + C, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(h, hipMemcpyAsync(&NCH, f.choff + C, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(h, hipStreamSynchronize(s));
@@ -332,22 +378,23 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     f.NCH = NCH;
     if (F == 0) {
         // nothing to draw: the white clear of the reference's output pass
-        sl::launch_fill_u8(f.image, 255, image_bytes, s);
+        if (carved) sl::launch_fill_rows_u8(f.image, 255, (size_t)f.W * 4, (size_t)f.pitch, (size_t)f.H, s);
+        else sl::launch_fill_u8(f.image, 255, image_bytes, s);
         for (int i = 2; i <= kStages; ++i) mark(i);
         if ((rc = check_launch(h, "clear"))) return rc;
         if (replay) return SL_OK;
         HIPCHK(h, hipStreamSynchronize(s));
-        h->last_F = 0; h->last_NCH = 0; h->last_NB = 0; h->sized = true;
+        if (!carved) { h->last_F = 0; h->last_NCH = 0; h->last_NB = 0; h->sized = true; }
         return SL_OK;
     }
-    if ((rc = ensure(h, h->b_ts, (size_t)F * 8)) || (rc = ensure(h, h->b_chains, (size_t)NCH * 6 + 16)) ||
-        (rc = ensure(h, h->b_chain_t, (size_t)F * 4 + 64)))  // padded: read in aligned 16-byte groups
+    if ((rc = ensure_in(h, A, A.b_ts, (size_t)F * 8)) || (rc = ensure_in(h, A, A.b_chains, (size_t)NCH * 6 + 16)) ||
+        (rc = ensure_in(h, A, A.b_chain_t, (size_t)F * 4 + 64)))  // padded: read in aligned 16-byte groups
         return rc;
-    f.ts = P_<int2>(h->b_ts);
-    f.chains = P_<uint32_t>(h->b_chains);
+    f.ts = P_<int2>(A.b_ts);
+    f.chains = P_<uint32_t>(A.b_chains);
     f.plist = f.chains + (size_t)NCH;
-    f.lcount = P_<uint32_t>(h->b_lcount);
-    f.chain_t = P_<float>(h->b_chain_t);
+    f.lcount = P_<uint32_t>(A.b_lcount);
+    f.chain_t = P_<float>(A.b_chain_t);
     mark(2);
     sl::launch_intersect(sc, f, s);
     if ((rc = check_launch(h, "intersect"))) return rc;
@@ -362,40 +409,40 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     const size_t rs_tiles = ((size_t)NB + 2047) / 2048;
     scan_need = sl::scan_tmp_elems_needed(
         std::max<size_t>({(size_t)C + 1, (size_t)P + 1, (size_t)NB + 1, 256 * rs_tiles + 1, (size_t)F + 1}));
-    if ((rc = ensure(h, h->b_binfo, (size_t)NB * 8)) || (rc = ensure(h, h->b_bcount, ((size_t)NB + 1) * 4)) ||
-        (rc = ensure(h, h->b_boff, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bwsum, ((size_t)NB + 1) * 4)) ||
-        (rc = ensure(h, h->b_bcarry, ((size_t)NB + 1) * 4)) ||
-        (rc = ensure(h, h->b_rec_meta, (size_t)F * 4)) || (rc = ensure(h, h->b_rec_mask, (size_t)F * 4 * NW)) ||
-        (rc = ensure(h, h->b_scratch, (size_t)F * 8)) || (rc = ensure(h, h->b_items, (size_t)F * 8)) ||
-        (rc = ensure(h, h->b_imask, (size_t)F * 8 * NW)) || (rc = ensure(h, h->b_icov, (size_t)F * 8)) || (rc = ensure(h, h->b_bicnt, ((size_t)NB + 1) * 4)) ||
-        (rc = ensure(h, h->b_bmcnt, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_bext, (size_t)NB * 4)) ||
-        (rc = ensure(h, h->b_big_list, (size_t)NB * 4)) || (rc = ensure(h, h->b_nz_flag, ((size_t)NB + 1) * 4)) ||
-        (rc = ensure(h, h->b_nz_off, ((size_t)NB + 1) * 4)) || (rc = ensure(h, h->b_nzk0, (size_t)NB * 4)) ||
-        (rc = ensure(h, h->b_nzk1, (size_t)NB * 4)) || (rc = ensure(h, h->b_nzv0, (size_t)NB * 4)) ||
-        (rc = ensure(h, h->b_nzv1, (size_t)NB * 4)) || (rc = ensure(h, h->b_rs_hist, 256 * rs_tiles * 4 + 16)) ||
-        (rc = ensure(h, h->b_rs_hoff, (256 * rs_tiles + 1) * 4 + 16)) || (rc = ensure(h, h->b_rowbeg, R * 4)) ||
-        (rc = ensure(h, h->b_rowend, R * 4)) || (rc = ensure(h, h->b_rrec, (size_t)NB * 16)) ||
-        (rc = ensure(h, h->b_btile, (size_t)NB * 8)) || (rc = ensure(h, h->b_rtile, (size_t)NB * 8)) ||
-        (rc = ensure(h, h->b_scan_tmp, scan_need * 4)))
+    if ((rc = ensure_in(h, A, A.b_binfo, (size_t)NB * 8)) || (rc = ensure_in(h, A, A.b_bcount, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_boff, ((size_t)NB + 1) * 4)) || (rc = ensure_in(h, A, A.b_bwsum, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_bcarry, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_rec_meta, (size_t)F * 4)) || (rc = ensure_in(h, A, A.b_rec_mask, (size_t)F * 4 * NW)) ||
+        (rc = ensure_in(h, A, A.b_scratch, (size_t)F * 8)) || (rc = ensure_in(h, A, A.b_items, (size_t)F * 8)) ||
+        (rc = ensure_in(h, A, A.b_imask, (carved && !f.keep_masks) ? 0 : (size_t)F * 8 * NW)) || (rc = ensure_in(h, A, A.b_icov, (size_t)F * 8)) || (rc = ensure_in(h, A, A.b_bicnt, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_bmcnt, ((size_t)NB + 1) * 4)) || (rc = ensure_in(h, A, A.b_bext, (size_t)NB * 4)) ||
+        (rc = ensure_in(h, A, A.b_big_list, (size_t)NB * 4)) || (rc = ensure_in(h, A, A.b_nz_flag, ((size_t)NB + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_nz_off, ((size_t)NB + 1) * 4)) || (rc = ensure_in(h, A, A.b_nzk0, (size_t)NB * 4)) ||
+        (rc = ensure_in(h, A, A.b_nzk1, (size_t)NB * 4)) || (rc = ensure_in(h, A, A.b_nzv0, (size_t)NB * 4)) ||
+        (rc = ensure_in(h, A, A.b_nzv1, (size_t)NB * 4)) || (rc = ensure_in(h, A, A.b_rs_hist, 256 * rs_tiles * 4 + 16)) ||
+        (rc = ensure_in(h, A, A.b_rs_hoff, (256 * rs_tiles + 1) * 4 + 16)) || (rc = ensure_in(h, A, A.b_rowbeg, R * 4)) ||
+        (rc = ensure_in(h, A, A.b_rowend, R * 4)) || (rc = ensure_in(h, A, A.b_rrec, (size_t)NB * 16)) ||
+        (rc = ensure_in(h, A, A.b_btile, (size_t)NB * 8)) || (rc = ensure_in(h, A, A.b_rtile, (size_t)NB * 8)) ||
+        (rc = ensure_in(h, A, A.b_scan_tmp, scan_need * 4)))
         return rc;
-    f.binfo = P_<int2>(h->b_binfo); f.bcount = P_<uint32_t>(h->b_bcount); f.boff = P_<uint32_t>(h->b_boff);
-    f.bwsum = P_<uint32_t>(h->b_bwsum); f.bcarry = P_<uint32_t>(h->b_bcarry);
-    f.rec_meta = P_<uint32_t>(h->b_rec_meta); f.rec_mask = P_<uint32_t>(h->b_rec_mask);
-    f.scratch_keys = P_<uint32_t>(h->b_scratch); f.items = P_<uint32_t>(h->b_items); f.imask = P_<uint32_t>(h->b_imask);
-    f.icov = P_<uint32_t>(h->b_icov);
+    f.binfo = P_<int2>(A.b_binfo); f.bcount = P_<uint32_t>(A.b_bcount); f.boff = P_<uint32_t>(A.b_boff);
+    f.bwsum = P_<uint32_t>(A.b_bwsum); f.bcarry = P_<uint32_t>(A.b_bcarry);
+    f.rec_meta = P_<uint32_t>(A.b_rec_meta); f.rec_mask = P_<uint32_t>(A.b_rec_mask);
+    f.scratch_keys = P_<uint32_t>(A.b_scratch); f.items = P_<uint32_t>(A.b_items); f.imask = P_<uint32_t>(A.b_imask);
+    f.icov = P_<uint32_t>(A.b_icov);
     // the rank array lives in the large-bucket scratch (used only later, by the resolve);
     // the winding scan's output lives in the item masks the resolve overwrites
     f.frank = f.scratch_keys;
-    f.bicnt = P_<uint32_t>(h->b_bicnt); f.bmcnt = P_<uint32_t>(h->b_bmcnt); f.bext = P_<uint32_t>(h->b_bext);
-    f.big_list = P_<uint32_t>(h->b_big_list); f.nz_flag = P_<uint32_t>(h->b_nz_flag); f.nz_off = P_<uint32_t>(h->b_nz_off);
-    f.nz_keys[0] = P_<uint32_t>(h->b_nzk0); f.nz_keys[1] = P_<uint32_t>(h->b_nzk1);
-    f.nz_vals[0] = P_<uint32_t>(h->b_nzv0); f.nz_vals[1] = P_<uint32_t>(h->b_nzv1);
-    f.rs_hist = P_<uint32_t>(h->b_rs_hist); f.rs_hoff = P_<uint32_t>(h->b_rs_hoff);
-    f.rowbeg = P_<uint32_t>(h->b_rowbeg); f.rowend = P_<uint32_t>(h->b_rowend);
-    f.rrec = P_<uint4>(h->b_rrec);
-    f.btile = P_<uint64_t>(h->b_btile);
-    f.rtile = P_<uint64_t>(h->b_rtile);
-    f.scan_tmp = P_<uint32_t>(h->b_scan_tmp);
+    f.bicnt = P_<uint32_t>(A.b_bicnt); f.bmcnt = P_<uint32_t>(A.b_bmcnt); f.bext = P_<uint32_t>(A.b_bext);
+    f.big_list = P_<uint32_t>(A.b_big_list); f.nz_flag = P_<uint32_t>(A.b_nz_flag); f.nz_off = P_<uint32_t>(A.b_nz_off);
+    f.nz_keys[0] = P_<uint32_t>(A.b_nzk0); f.nz_keys[1] = P_<uint32_t>(A.b_nzk1);
+    f.nz_vals[0] = P_<uint32_t>(A.b_nzv0); f.nz_vals[1] = P_<uint32_t>(A.b_nzv1);
+    f.rs_hist = P_<uint32_t>(A.b_rs_hist); f.rs_hoff = P_<uint32_t>(A.b_rs_hoff);
+    f.rowbeg = P_<uint32_t>(A.b_rowbeg); f.rowend = P_<uint32_t>(A.b_rowend);
+    f.rrec = P_<uint4>(A.b_rrec);
+    f.btile = P_<uint64_t>(A.b_btile);
+    f.rtile = P_<uint64_t>(A.b_rtile);
+    f.scan_tmp = P_<uint32_t>(A.b_scan_tmp);
 
     sl::launch_bucket_info(sc, f, s);
     if ((rc = check_launch(h, "bucket_info"))) return rc;
@@ -412,7 +459,7 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     mark(6);
     sl::launch_order_rows(sc, f, s);
     if ((rc = check_launch(h, "order_rows"))) return rc;
-    h->nz_vals_sorted = f.nz_vals[0];
+    if (!carved) h->nz_vals_sorted = f.nz_vals[0];
     mark(7);
     sl::launch_composite(sc, tb, f, s);
     if ((rc = check_launch(h, "composite"))) return rc;
@@ -420,18 +467,18 @@ int render(sl_handle *h, hipStream_t s, bool replay = false) {
     if (replay) return SL_OK;
 
     // element counts (M, S) and the error flag
-    unsigned long long *red = (unsigned long long *)(P_<uint32_t>(h->b_small) + 4);
+    unsigned long long *red = (unsigned long long *)(P_<uint32_t>(A.b_small) + 4);
     sl::launch_fill_u32((uint32_t *)red, 0u, 4, s);
     sl::launch_count_reduce(f.bmcnt, f.bicnt, NB, red, s);
     uint32_t small[8];
-    HIPCHK(h, hipMemcpyAsync(small, h->b_small.p, 32, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipMemcpyAsync(small, A.b_small.p, 32, hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
     const unsigned long long M = *(unsigned long long *)&small[4], MS = *(unsigned long long *)&small[6];
     h->counts.merged_fragment = (int64_t)M;
     h->counts.span = (int64_t)(MS - M);
     if (small[1] & 1u) return fail(h, SL_E_HIP, "internal error: fragment row outside its path's bucket range");
     if (small[1] & 2u) return fail(h, SL_E_UNSUPPORTED, "a (path, row) bucket holds more than 65536 fragments");
-    h->last_F = F; h->last_NCH = NCH; h->last_NB = NB; h->sized = true;
+    if (!carved) { h->last_F = F; h->last_NCH = NCH; h->last_NB = NB; h->sized = true; }
 
     if (h->timing) {
         for (int i = 0; i < kStages; ++i) {
@@ -543,6 +590,12 @@ int sl_create(int device, sl_handle **out) {
         delete h;
         return SL_E_HIP;
     }
+    if (hipEventCreateWithFlags(&h->switch_ev, hipEventDisableTiming) != hipSuccess) {
+        for (int k = 0; k <= kStages; ++k) (void)hipEventDestroy(h->ev[k]);
+        (void)hipEventDestroy(h->done_ev);
+        delete h;
+        return SL_E_HIP;
+    }
     h->events_ready = true;
     *out = h;
     return SL_OK;
@@ -552,19 +605,20 @@ int sl_destroy(sl_handle *h) {
     if (!h) return SL_OK;
     (void)hipSetDevice(h->device);
     (void)hipDeviceSynchronize();
-    DevBuf *bufs[] = {&h->b_pcnt, &h->b_nchain, &h->b_choff, &h->b_chains, &h->b_chain_t, &h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->cinfo, &h->frule, &h->finfo,
-                      &h->ptable[0], &h->ptable[1], &h->ptable[2], &h->ctab[0], &h->ctab[1], &h->b_tvtx, &h->b_pflags,
-                      &h->b_prow, &h->b_ccuts, &h->b_cncut, &h->b_cfrag, &h->b_coff, &h->b_ts, &h->b_pnb, &h->b_pbase,
-                      &h->b_binfo, &h->b_bcount, &h->b_boff, &h->b_bwsum, &h->b_bcarry, &h->b_rec_meta,
-                      &h->b_rec_mask, &h->b_scratch, &h->b_items, &h->b_imask, &h->b_icov, &h->b_bicnt, &h->b_bmcnt, &h->b_bext,
-                      &h->b_big_list, &h->b_small, &h->b_nz_flag, &h->b_nz_off, &h->b_nzk0, &h->b_nzk1, &h->b_nzv0,
-                      &h->b_nzv1, &h->b_rs_hist, &h->b_rs_hoff, &h->b_rowbeg, &h->b_rowend, &h->b_image, &h->b_scan_tmp,
-                      &h->b_rrec, &h->b_btile, &h->b_rtile, &h->b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
+    DevBuf *bufs[] = {&h->A.b_pcnt, &h->A.b_nchain, &h->A.b_choff, &h->A.b_chains, &h->A.b_chain_t, &h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->cinfo, &h->frule, &h->finfo,
+                      &h->ptable[0], &h->ptable[1], &h->ptable[2], &h->ctab[0], &h->ctab[1], &h->A.b_tvtx, &h->A.b_pflags,
+                      &h->A.b_prow, &h->A.b_ccuts, &h->A.b_cncut, &h->A.b_cfrag, &h->A.b_coff, &h->A.b_ts, &h->A.b_pnb, &h->A.b_pbase,
+                      &h->A.b_binfo, &h->A.b_bcount, &h->A.b_boff, &h->A.b_bwsum, &h->A.b_bcarry, &h->A.b_rec_meta,
+                      &h->A.b_rec_mask, &h->A.b_scratch, &h->A.b_items, &h->A.b_imask, &h->A.b_icov, &h->A.b_bicnt, &h->A.b_bmcnt, &h->A.b_bext,
+                      &h->A.b_big_list, &h->A.b_small, &h->A.b_nz_flag, &h->A.b_nz_off, &h->A.b_nzk0, &h->A.b_nzk1, &h->A.b_nzv0,
+                      &h->A.b_nzv1, &h->A.b_rs_hist, &h->A.b_rs_hoff, &h->A.b_rowbeg, &h->A.b_rowend, &h->A.b_image, &h->A.b_scan_tmp,
+                      &h->A.b_rrec, &h->A.b_btile, &h->A.b_rtile, &h->A.b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
     for (DevBuf *b : bufs) release(*b);
     if (h->stage) (void)hipHostFree(h->stage);
     if (h->events_ready) {
         for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(h->ev[i]);
         (void)hipEventDestroy(h->done_ev);
+        (void)hipEventDestroy(h->switch_ev);
     }
     delete h;
     return SL_OK;
@@ -576,7 +630,8 @@ int sl_set_samples(sl_handle *h, int samples) {
         return fail(h, SL_E_UNSUPPORTED, "samples must be 8, 16 or 32 (got %d)", samples);
     h->spp = samples;
     h->sized = false;
-    return SL_OK;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    return ensure_tables(h);  // built here, so a later allocation-free frame finds them
 }
 
 int sl_set_multisample_output(sl_handle *h, int enable) {
@@ -590,7 +645,8 @@ int sl_enable_srgb_correction(sl_handle *h, int enable) {
     if (!h) return SL_E_ARG;
     h->srgb = enable ? 1 : 0;
     h->sized = false;
-    return SL_OK;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    return ensure_tables(h);
 }
 
 int sl_set_input_transform(sl_handle *h, const float m[16]) {
@@ -796,7 +852,7 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
     HIPCHK(h, hipStreamSynchronize(nullptr));
     h->P = P; h->C = C; h->V = (P == 0) ? 0 : V;
     h->has_scene = true;
-    return SL_OK;
+    return ensure_tables(h);
 }
 
 int sl_clear(sl_handle *h) {
@@ -815,45 +871,60 @@ int sl_rasterize(sl_handle *h, void *stream) {
     // (every kernel runs again; only its counts and sizes are taken over), queued on the
     // stream, and the call returns at once.  Step timing and parity dumps need the host
     // reads of a full frame, so they keep every frame synchronous.
-    // A frame still queued on another stream shares this frame's buffers: a replay waits
-    // for it on the device, a full frame (which may reallocate) on the host.
+    // A frame still queued shares this frame's buffers.  On the same stream the order is
+    // the stream's.  On another stream, this frame comes after everything queued on the
+    // previous frame's stream so far (that frame, and any reads of its image queued after
+    // it): on the device for a replay, on the host for a full frame (which may reallocate).
     const hipStream_t s = (hipStream_t)stream;
     const bool replay = h->sized && !h->timing && !h->debug;
-    if (h->pending && (!replay || s != h->pending_stream)) {
-        if (replay) {
-            HIPCHK(h, hipStreamWaitEvent(s, h->done_ev, 0));
-        } else {
-            const int src = settle(h);
-            if (src) return src;
-        }
+    if (h->pending && s != h->pending_stream) {
+        HIPCHK(h, hipEventRecord(h->switch_ev, h->pending_stream));
+        if (replay) HIPCHK(h, hipStreamWaitEvent(s, h->switch_ev, 0));
+        else HIPCHK(h, hipEventSynchronize(h->switch_ev));
+        h->pending = false;
+    } else if (h->pending && !replay) {
+        const int src = settle(h);
+        if (src) return src;
     }
-    const int rc = render(h, s, replay);
-    h->pending = false;
-    if (rc) return rc;
-    if (replay) {
-        HIPCHK(h, hipEventRecord(h->done_ev, s));
+    const int rc = render(h, s, replay, h->A);
+    // Whether or not the frame succeeded, some of its kernels may be queued on s: later
+    // uploads and frames must wait for them as for a completed replay.
+    if (hipEventRecord(h->done_ev, s) == hipSuccess) {
         h->pending = true;
         h->pending_stream = s;
+    } else {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(s);
+        h->pending = false;
     }
+    return rc;
+}
+
+// The image of the last frame, if the output size is still the one it was rendered at.
+static int check_image(sl_handle *h) {
+    if (!h->A.b_image.p || h->img_W <= 0) return fail(h, SL_E_STATE, "no image rendered yet");
+    if (h->img_W != h->W || h->img_H != h->H)
+        return fail(h, SL_E_STATE, "output size changed to %dx%d since the last frame (%dx%d): rasterize first",
+                    h->W, h->H, h->img_W, h->img_H);
     return SL_OK;
 }
 
 int sl_output_device_ptr(sl_handle *h, void **dev_ptr, int *pitch_bytes) {
     if (!h || !dev_ptr || !pitch_bytes) return SL_E_ARG;
-    if (!h->b_image.p) return fail(h, SL_E_STATE, "no image rendered yet");
-    *dev_ptr = h->b_image.p;
-    *pitch_bytes = h->W * 4;
+    if (int rc = check_image(h)) return rc;
+    *dev_ptr = h->A.b_image.p;
+    *pitch_bytes = h->img_W * 4;
     return SL_OK;
 }
 
 int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags) {
     if (!h || !host_rgba) return SL_E_ARG;
-    if (!h->b_image.p) return fail(h, SL_E_STATE, "no image rendered yet");
-    int rc = settle(h);
+    int rc = check_image(h);
     if (rc) return rc;
-    const size_t row = (size_t)h->W * 4;
-    HIPCHK(h, hipMemcpy(host_rgba, h->b_image.p, row * h->H, hipMemcpyDeviceToHost));
-    if (flags & 1) flip_rows_alpha(host_rgba, h->W, h->H);
+    if ((rc = settle(h))) return rc;
+    const size_t row = (size_t)h->img_W * 4;
+    HIPCHK(h, hipMemcpy(host_rgba, h->A.b_image.p, row * h->img_H, hipMemcpyDeviceToHost));
+    if (flags & 1) flip_rows_alpha(host_rgba, h->img_W, h->img_H);
     return SL_OK;
 }
 
@@ -949,10 +1020,10 @@ int sl_debug_dump(sl_handle *h, const char *name, void *host_dst, size_t capacit
     size_t n = 0;
     std::string k(name);
     if (k == "image") {
-        n = (size_t)h->W * h->H * 4;
-        if (!h->b_image.p) return fail(h, SL_E_STATE, "no image");
+        if ((rc = check_image(h))) return rc;
+        n = (size_t)h->img_W * h->img_H * 4;
         *nbytes = n;
-        if (host_dst && capacity >= n) HIPCHK(h, hipMemcpy(host_dst, h->b_image.p, n, hipMemcpyDeviceToHost));
+        if (host_dst && capacity >= n) HIPCHK(h, hipMemcpy(host_dst, h->A.b_image.p, n, hipMemcpyDeviceToHost));
         return SL_OK;
     }
     if (k == "pixel_table") {
@@ -999,6 +1070,7 @@ struct sl_batch {
     int shape[4] = {0, 0, 4, 256};        // resolve grid, class grid, compositor group, fragment block
     // the last frame issued by sl_batch_render(_eager), possibly still running
     hipEvent_t done_ev = nullptr;
+    hipEvent_t switch_ev = nullptr;       // recorded on last_stream when a frame moves on
     hipStream_t last_stream = nullptr;
     bool pending = false;
     std::string err;
@@ -1017,10 +1089,14 @@ int bfail(sl_batch *b, int code, const char *fmt, ...) {
 }
 
 // Every frame of a batch uses the same buffers.  A frame on another stream than the last
-// one waits for it on the device; tuning and recapture wait on the host.
+// one waits on the device for everything queued on that stream so far (the last frame and
+// any reads of its images queued after it); tuning and recapture wait on the host.
 int batch_order(sl_batch *b, hipStream_t s) {
-    if (b->pending && s != b->last_stream && hipStreamWaitEvent(s, b->done_ev, 0) != hipSuccess)
-        return bfail(b, SL_E_HIP, "hipStreamWaitEvent failed");
+    if (b->pending && s != b->last_stream) {
+        if (hipEventRecord(b->switch_ev, b->last_stream) != hipSuccess ||
+            hipStreamWaitEvent(s, b->switch_ev, 0) != hipSuccess)
+            return bfail(b, SL_E_HIP, "ordering after the previous stream failed");
+    }
     return SL_OK;
 }
 
@@ -1055,7 +1131,7 @@ void apply_shape(sl_batch *b) {
 
 int enqueue_all(sl_batch *b, hipStream_t s, bool replay) {
     for (sl_handle *h : b->grp) {
-        const int rc = render(h, s, replay);
+        const int rc = render(h, s, replay, h->A);
         if (rc) return bfail(b, rc, "%s", h->err.c_str());
     }
     return SL_OK;
@@ -1179,6 +1255,7 @@ int sl_batch_create(int device, const sl_scene *scenes, const int *widths, const
     }
     if (!rc && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) rc = SL_E_HIP;
     if (!rc && hipEventCreateWithFlags(&b->done_ev, hipEventDisableTiming) != hipSuccess) rc = SL_E_HIP;
+    if (!rc && hipEventCreateWithFlags(&b->switch_ev, hipEventDisableTiming) != hipSuccess) rc = SL_E_HIP;
     if (!rc) {
         // one eager frame sizes every buffer; the graph then replays it without host syncs
         apply_shape(b);
@@ -1202,6 +1279,7 @@ int sl_batch_destroy(sl_batch *b) {
     for (sl_handle *h : b->grp) sl_destroy(h);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     if (b->done_ev) (void)hipEventDestroy(b->done_ev);
+    if (b->switch_ev) (void)hipEventDestroy(b->switch_ev);
     delete b;
     return SL_OK;
 }
@@ -1304,7 +1382,7 @@ int sl_batch_element_number(sl_batch *b, sl_counts *out) {
 int sl_batch_output_device_ptr(sl_batch *b, int scene, void **dev_ptr, int *pitch_bytes) {
     if (!b || scene < 0 || scene >= b->n || !dev_ptr || !pitch_bytes) return SL_E_ARG;
     sl_handle *h = b->grp[b->scene_grp[scene]];
-    *dev_ptr = (uint8_t *)h->b_image.p + b->scene_off[scene];
+    *dev_ptr = (uint8_t *)h->A.b_image.p + b->scene_off[scene];
     *pitch_bytes = 4 * b->scene_w[scene];
     return SL_OK;
 }
@@ -1317,7 +1395,7 @@ int sl_batch_read_output(sl_batch *b, int scene, uint8_t *host_rgba, int flags) 
     sl_handle *h = b->grp[b->scene_grp[scene]];
     const int W = b->scene_w[scene], H = b->scene_h[scene];
     const size_t bytes = (size_t)W * H * 4;
-    const hipError_t e = hipMemcpy(host_rgba, (uint8_t *)h->b_image.p + b->scene_off[scene], bytes,
+    const hipError_t e = hipMemcpy(host_rgba, (uint8_t *)h->A.b_image.p + b->scene_off[scene], bytes,
                                    hipMemcpyDeviceToHost);
     if (e != hipSuccess) return bfail(b, SL_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
     if (flags & 1) flip_rows_alpha(host_rgba, W, H);

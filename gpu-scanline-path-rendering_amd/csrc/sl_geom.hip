@@ -99,6 +99,12 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, size_t n) {
     if (i < n) p[i] = v;
 }
 
+// rows of `row_bytes` bytes, `pitch` apart (the bytes between rows are left alone)
+__global__ void k_fill_rows_u8(uint8_t *p, uint8_t v, size_t row_bytes, size_t pitch, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[(i / row_bytes) * pitch + i % row_bytes] = v;
+}
+
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 // ------------------------------------------------------------------------------------
@@ -886,6 +892,12 @@ void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+void launch_fill_rows_u8(uint8_t *p, uint8_t v, size_t row_bytes, size_t pitch, size_t rows, hipStream_t s) {
+    const size_t n = row_bytes * rows;
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_rows_u8, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, row_bytes, pitch, n);
 }
 
 void launch_path_init(const SceneDev &sc, Frame &f, hipStream_t s) {

@@ -33,6 +33,8 @@ class DiffRenderer:
         c = a.fill_color.astype(np.uint32)
         rgba = np.stack([(c >> (8 * k)) & 0xFF for k in range(4)], axis=1).astype(np.float32) / 255.0
         self.colors0 = torch.tensor(rgba, device=self.device)
+        # the library keeps one forward's state for the adjoint; every forward replaces it
+        self.generation = 0
 
     def _chk(self, rc):
         if rc:
@@ -63,6 +65,7 @@ class DiffRenderer:
         points, colors = points.contiguous(), colors.contiguous()
         img = torch.empty((self.height, self.width, 4), dtype=torch.float32, device=self.device)
         stream = torch.cuda.current_stream(self.device).cuda_stream
+        self.generation += 1
         self._chk(self._L.sl_diff_forward(self._h, points.data_ptr(), colors.data_ptr(), img.data_ptr(),
                                           1 if keep else 0, ctypes.c_void_p(stream)))
         self._kept = (points, colors) if keep else None
@@ -88,14 +91,27 @@ class DiffRenderer:
 
 
 class _Render(torch.autograd.Function):
+    """The library holds the state of one forward per renderer: a backward is only valid
+    for the renderer's latest forward, and raises if a later forward replaced it."""
+
     @staticmethod
     def forward(ctx, points, colors, renderer):
         ctx.renderer = renderer
-        return renderer.forward_raw(points, colors, keep=True)
+        img = renderer.forward_raw(points, colors, keep=True)
+        ctx.generation = renderer.generation
+        # the adjoint reads the forward's points and colours: keep them, and let autograd
+        # notice in-place changes before the backward
+        ctx.save_for_backward(points, colors)
+        return img
 
     @staticmethod
     def backward(ctx, grad_image):
-        gp, gc = ctx.renderer.backward_raw(grad_image)
+        r = ctx.renderer
+        if r.generation != ctx.generation:
+            raise RuntimeError("DiffRenderer.render: a later forward (render or forward_raw) replaced the state "
+                               "this backward needs; one pending backward per renderer")
+        ctx.saved_tensors  # raises if the inputs were modified in place since the forward
+        gp, gc = r.backward_raw(grad_image)
         return gp, gc, None
 
 

@@ -136,15 +136,21 @@ int sl_clear(sl_handle *h);
  * once (step timing and debug dumps keep every frame synchronous).  sl_read_output waits
  * for such a frame; users of sl_output_device_ptr order their work after it on `stream`.
  * A queued frame shares the handle's buffers with the next one: sl_add_vg, sl_debug_dump
- * and a full frame wait for it on the host, and a queued frame on another stream waits for
- * it on the device. */
+ * and a full frame on the same stream wait for it on the host.  A frame issued on another
+ * stream comes after everything queued on the previous frame's stream up to the call (that
+ * frame and the caller's reads of its image queued after it): on the device for a queued
+ * frame, on the host for a full one.  A frame that fails is still waited for like a queued
+ * one, since some of its kernels may have been issued. */
 int sl_rasterize(sl_handle *h, void *stream);
 
-/* Device pointer and row pitch (bytes) of the handle's RGBA8 output image. */
+/* Device pointer and row pitch (bytes) of the handle's RGBA8 output image.  Fails with
+ * SL_E_STATE when the output size changed since the last frame (the image then still has
+ * the old size); the same holds for sl_read_output and the "image" dump. */
 int sl_output_device_ptr(sl_handle *h, void **dev_ptr, int *pitch_bytes);
 
-/* Copy the last image to host memory (width*height*4 bytes).  flags bit 0: flip rows
- * and force alpha to 255 as saveOutputToFile does (ras_base.cpp:1249-1262). */
+/* Copy the last image to host memory (width*height*4 bytes of the size it was rendered
+ * at).  flags bit 0: flip rows and force alpha to 255 as saveOutputToFile does
+ * (ras_base.cpp:1249-1262). */
 int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags);
 
 /* elementNumber() (ras_base.h:207). */
@@ -203,8 +209,9 @@ int sl_debug_requantize_sweep(sl_handle *h, int channel, uint32_t bits_lo, uint3
  * sl_batch_tune times candidate launch shapes {per-bucket resolve grid, size-class
  * grid, compositor bucket group, fragment block size} on this batch, keeps the fastest
  * and recaptures.  Images are read per scene.  A frame issued on another stream than the
- * previous one waits for it on the device; tuning, a shape change and the read-back wait
- * for it on the host. */
+ * previous one waits on the device for everything queued on that stream up to the call
+ * (the previous frame and the caller's reads of its images); tuning, a shape change and
+ * the read-back wait for it on the host. */
 typedef struct sl_batch sl_batch;
 int sl_batch_create(int device, const sl_scene *scenes, const int *widths, const int *heights,
                     const int *samples, int n_scenes, int srgb, sl_batch **out);

@@ -195,3 +195,25 @@ def test_forward_and_backward_on_alternating_streams(setup):
         assert torch.equal(gi, ri)
         assert torch.allclose(gp, rp, rtol=1e-5, atol=1e-6 * float(rp.abs().max()))
         assert torch.allclose(gc, rc, rtol=1e-5, atol=1e-6 * float(rc.abs().max()))
+
+
+def test_backward_of_a_replaced_forward_raises(setup):
+    """The library keeps one forward's state: a backward of an earlier render, after a
+    later render replaced that state, raises instead of returning the later render's
+    gradients; the latest render's backward still works."""
+    scene, W, H, r, uniq, index, tgt, _, _ = setup
+    p1 = r.points0.clone().requires_grad_(True)
+    p2 = (r.points0 + 0.25).requires_grad_(True)
+    col = r.colors0.clone().requires_grad_(True)
+    loss1 = ((r.render(p1, col) - tgt) ** 2).sum()
+    loss2 = ((r.render(p2, col) - tgt) ** 2).sum()
+    with pytest.raises(RuntimeError, match="replaced the state"):
+        loss1.backward()
+    loss2.backward()
+    assert p2.grad is not None and torch.isfinite(p2.grad).all() and p2.grad.abs().sum() > 0
+    # an evaluation forward between a render and its backward replaces the state too
+    p3 = r.points0.clone().requires_grad_(True)
+    loss3 = ((r.render(p3, col.detach()) - tgt) ** 2).sum()
+    r.forward_raw(r.points0, r.colors0, keep=False)
+    with pytest.raises(RuntimeError, match="replaced the state"):
+        loss3.backward()

@@ -501,3 +501,30 @@ def test_image_dump_after_queued_frames(rasterizer):
     img = r.dump("image").reshape(H, W, 4)
     assert np.abs(img.astype(np.int32) - ref.astype(np.int32)).max() <= 1
     torch.cuda.synchronize()
+
+
+def test_output_after_resize_needs_a_new_frame():
+    """After resizeOutputBuffer the last image has the old size: reading it (host copy,
+    device pointer, image dump) fails with SL_E_STATE until the next frame, which then
+    reads back at the new size."""
+    r = scanline.VGRasterizer(0)
+    try:
+        scene = scanline.generate(scanline.scene.KIND_MIXED, 96, 64, 20, 0x5EED0600)
+        render_gpu(r, scene, 96, 64, 8)
+        r.dumpDebugData(False)
+        assert r.output().shape == (64, 96, 4)
+        for w, h in ((160, 120), (40, 30)):
+            r.resizeOutputBuffer(w, h)
+            with pytest.raises(scanline.ScanlineError) as e:
+                r.output()
+            assert e.value.code == scanline._abi.SL_E_STATE
+            with pytest.raises(scanline.ScanlineError):
+                r.outputDevicePtr()
+            with pytest.raises(scanline.ScanlineError):
+                r.dump("image")
+            r.rasterize()
+            ref = pyoracle.run(scene, w, h, 8, srgb=True).get("image").reshape(h, w, 4)
+            np.testing.assert_array_equal(r.output(), ref)
+            assert r.outputDevicePtr()[1] == w * 4
+    finally:
+        r.close()
