@@ -103,6 +103,8 @@ struct sl_handle {
     hipEvent_t switch_ev = nullptr;
     // size of the image the last frame wrote (the output size may change since)
     int img_W = 0, img_H = 0;
+    // device address of this GPU's static scratch (the allocation-free frame)
+    void *noalloc_scratch = nullptr;
     int64_t NB = 0;
     uint32_t *nz_vals_sorted = nullptr;
 };
@@ -161,8 +163,8 @@ int ensure_in(sl_handle *h, Arena &A, DevBuf &b, size_t bytes) {
     if (at > A.carve_cap || bytes > A.carve_cap - at)
         return fail(h, SL_E_UNSUPPORTED,
                     "the allocation-free frame needs more than its %zu MiB of static scratch for this scene "
-                    "(%zu bytes in use, %zu more asked); render it with a handle's own arena",
-                    A.carve_cap >> 20, at, bytes);
+                    "(%zu bytes in use, %zu more asked); render it with sl_rasterize",
+                    sl::noalloc_scratch_bytes() >> 20, at, bytes);
     b.p = A.carve_base + at;
     b.cap = bytes;
     A.carve_off = at + bytes;
@@ -597,6 +599,15 @@ int sl_create(int device, sl_handle **out) {
         return SL_E_HIP;
     }
     h->events_ready = true;
+    // set-up of the allocation-free frame: this GPU's address of the static scratch
+    if (sl::noalloc_scratch(device, &h->noalloc_scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        for (int k = 0; k <= kStages; ++k) (void)hipEventDestroy(h->ev[k]);
+        (void)hipEventDestroy(h->done_ev);
+        (void)hipEventDestroy(h->switch_ev);
+        delete h;
+        return SL_E_HIP;
+    }
     *out = h;
     return SL_OK;
 }
@@ -898,6 +909,33 @@ int sl_rasterize(sl_handle *h, void *stream) {
         h->pending = false;
     }
     return rc;
+}
+
+int sl_rasterize_noalloc(sl_handle *h, void *dev_rgba, int pitch_bytes, void *stream) {
+    if (!h) return SL_E_ARG;
+    if (!dev_rgba) return fail(h, SL_E_ARG, "output image pointer is NULL");
+    if (h->W > 0 && pitch_bytes < h->W * 4)
+        return fail(h, SL_E_ARG, "pitch %d is below width * 4 = %d", pitch_bytes, h->W * 4);
+    if (h->debug) return fail(h, SL_E_STATE, "debug dumps allocate: disable them for the allocation-free frame");
+    if (!h->noalloc_scratch) return fail(h, SL_E_STATE, "no static scratch on this device");
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    const hipStream_t s = (hipStream_t)stream;
+    // the handle's scene is read by a queued frame too (read only); order after it anyway,
+    // so that the counts this frame reports are the last ones
+    if (h->pending && s != h->pending_stream) {
+        HIPCHK(h, hipEventRecord(h->switch_ev, h->pending_stream));
+        HIPCHK(h, hipStreamWaitEvent(s, h->switch_ev, 0));
+    }
+    Arena A;
+    A.carve_base = (uint8_t *)h->noalloc_scratch;
+    A.carve_cap = sl::noalloc_scratch_bytes() - ((size_t)1 << 20);  // 1 MiB kept free at the end
+    const int rc = render(h, s, false, A, (uint8_t *)dev_rgba, pitch_bytes);
+    if (rc) {
+        // kernels queued before the failure read the scratch: let them drain
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    return SL_OK;
 }
 
 // The image of the last frame, if the output size is still the one it was rendered at.

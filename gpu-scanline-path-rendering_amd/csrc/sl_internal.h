@@ -160,6 +160,11 @@ void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launch_fill_rows_u8(uint8_t *p, uint8_t v, size_t row_bytes, size_t pitch, size_t rows, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 
+// static scratch of the allocation-free frame (sl_noalloc.hip): its size, and its device
+// address on `device` (which must be current; looked up once per device)
+size_t noalloc_scratch_bytes();
+hipError_t noalloc_scratch(int device, void **out);
+
 // debug (parity dumps)
 void launch_debug_fragments(const SceneDev &sc, const Tables &t, Frame &f, int32_t *yx, int32_t *pid,
                             int8_t *wn, uint32_t *mask4, hipStream_t s);

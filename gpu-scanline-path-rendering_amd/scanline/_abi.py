@@ -18,7 +18,7 @@ FILL_NONZERO, FILL_EVEN_ODD = 0, 1
 EXPORTS = (
     "sl_create", "sl_destroy", "sl_set_samples", "sl_set_multisample_output",
     "sl_enable_srgb_correction", "sl_set_input_transform", "sl_resize_output_buffer",
-    "sl_add_vg", "sl_clear", "sl_rasterize", "sl_output_device_ptr", "sl_read_output",
+    "sl_add_vg", "sl_clear", "sl_rasterize", "sl_rasterize_noalloc", "sl_output_device_ptr", "sl_read_output",
     "sl_element_number", "sl_enable_step_timing", "sl_step_timing", "sl_step_name",
     "sl_pixel_count", "sl_last_error", "sl_debug_enable", "sl_debug_dump",
     "sl_debug_requantize", "sl_debug_requantize_sweep", "sl_batch_create", "sl_batch_destroy",
@@ -85,6 +85,7 @@ def lib():
         "sl_add_vg": (i, [vp, ctypes.POINTER(SlScene)]),
         "sl_clear": (i, [vp]),
         "sl_rasterize": (i, [vp, vp]),
+        "sl_rasterize_noalloc": (i, [vp, vp, i, vp]),
         "sl_output_device_ptr": (i, [vp, ctypes.POINTER(vp), ctypes.POINTER(i)]),
         "sl_read_output": (i, [vp, vp, i]),
         "sl_element_number": (i, [vp, ctypes.POINTER(SlCounts)]),

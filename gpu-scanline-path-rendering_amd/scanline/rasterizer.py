@@ -97,6 +97,13 @@ class VGRasterizer:
         """stream: None or an integer hipStream_t (e.g. torch.cuda.current_stream().cuda_stream)."""
         self._chk(self._L.sl_rasterize(self._h, ctypes.c_void_p(stream or 0)))
 
+    def rasterizeNoAlloc(self, dev_ptr, pitch, stream=None):
+        """The allocation-free frame (no reference counterpart): renders into the caller's
+        device image at `dev_ptr` (rows of `pitch` bytes) with every intermediate array
+        in the library's static per-GPU scratch; see sl_rasterize_noalloc."""
+        self._chk(self._L.sl_rasterize_noalloc(self._h, ctypes.c_void_p(dev_ptr), int(pitch),
+                                               ctypes.c_void_p(stream or 0)))
+
     def saveOutputToFile(self, fileName):
         """saveOutputToFile (ras_base.cpp:1221-1277): the image with rows flipped and alpha
         forced to 255, as a PNG file."""

@@ -143,6 +143,21 @@ int sl_clear(sl_handle *h);
  * one, since some of its kernels may have been issued. */
 int sl_rasterize(sl_handle *h, void *stream);
 
+/* The frame of sl_rasterize, allocating nothing: no counterpart in the reference.  It is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The handle's scene and settings are
+ * rendered into the caller's device image `dev_rgba` (height rows of `pitch_bytes` >=
+ * width * 4 bytes) on `stream`; every intermediate array lives in one static device
+ * scratch of 256 MiB per GPU that the library holds until the process ends (its address
+ * is looked up at sl_create).  A scene needing more fails with SL_E_UNSUPPORTED and a
+ * message giving the size; render it with sl_rasterize.  The mask and colour tables are
+ * built by sl_add_vg, sl_set_samples and sl_enable_srgb_correction (which may allocate),
+ * and debug dumps must be off (SL_E_STATE otherwise).  Like a full sl_rasterize frame it
+ * reads its counts back and returns when the frame is done.  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (several host threads, several
+ * handles) race and must be serialised by the caller. */
+int sl_rasterize_noalloc(sl_handle *h, void *dev_rgba, int pitch_bytes, void *stream);
+
 /* Device pointer and row pitch (bytes) of the handle's RGBA8 output image.  Fails with
  * SL_E_STATE when the output size changed since the last frame (the image then still has
  * the old size); the same holds for sl_read_output and the "image" dump. */
