@@ -221,6 +221,10 @@ extern "C" int sl_scene_view(const sl_scene_obj *o, sl_scene *s) {
     s->curve_arc_w = o->cw.data();
     s->fill_rule = o->frule.data();
     s->fill_color = o->fcolor.data();
+    s->gradient_ramp = nullptr;
+    s->gradient_ramp_size = 0;
+    s->gradient_table = nullptr;
+    s->gradient_table_size = 0;
     return SL_OK;
 }
 

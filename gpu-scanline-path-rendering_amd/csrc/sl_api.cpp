@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -78,6 +79,10 @@ struct sl_handle {
 
     // scene (device)
     DevBuf vtx, vpath, cvpos, ctype, cpath, cw, crev, cinfo, frule, finfo;
+    // gradient textures of the scene: ramp texels (rows of 1024) and 12-word table slots
+    DevBuf gramp, gtable;
+    uint32_t gslots = 0, gramp_rows = 0;
+    bool has_grad = false;  // some path's fill word names a gradient slot
     // pinned host staging of the scene upload, kept and grown across sl_add_vg calls
     void *stage = nullptr;
     size_t stage_cap = 0;
@@ -284,6 +289,59 @@ uint32_t read_u32(sl_handle *h, const uint32_t *dptr, hipStream_t s, int &rc) {
 // One frame.  `replay` re-issues the work of the previous eager render of the same scene
 // without any host synchronisation (buffer sizes and counts are taken from that render),
 // so that the launch sequence can be captured into a HIP graph or timed back to back.
+// The gradient paint's inverse projection, as the reference's driver sets it
+// (updateRasterizerTransform, vg_main.cpp:117-143): with R the inverse of the input
+// transform, a window point s maps to rp + a / rd.z * rd, rd = rx s.x + ry s.y + rw, where
+// rp = R (W/2, H/2, -min(W, H), 1), rx, ry = R's first two columns, rw = R's last column -
+// rp and a = -rp.z: the ray from the eye through s meets the scene plane z = 0.  Computed
+// in double (Gauss-Jordan with partial pivoting) and rounded to float once; a singular
+// transform gives zeros.
+void inverse_projection(const float m[16], int W, int H, float ctx[13]) {
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            a[r][c] = (double)m[4 * c + r];  // column-major input
+            a[r][4 + c] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) {
+            for (int i = 0; i < 13; ++i) ctx[i] = 0.f;
+            return;
+        }
+        if (piv != col)
+            for (int c = 0; c < 8; ++c) std::swap(a[col][c], a[piv][c]);
+        const double d = a[col][col];
+        for (int c = 0; c < 8; ++c) a[col][c] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f != 0.0)
+                for (int c = 0; c < 8; ++c) a[r][c] -= f * a[col][c];
+        }
+    }
+    auto mul = [&](const double v[4], double o[4]) {
+        for (int r = 0; r < 4; ++r) o[r] = a[r][4] * v[0] + a[r][5] * v[1] + a[r][6] * v[2] + a[r][7] * v[3];
+    };
+    const double w = (double)W, h = (double)H;
+    const double vp[4] = {w / 2, h / 2, -std::min(w, h), 1.0}, ex[4] = {1, 0, 0, 0}, ey[4] = {0, 1, 0, 0},
+                 ew[4] = {0, 0, 0, 1};
+    double rp[4], rx[4], ry[4], rw[4];
+    mul(vp, rp);
+    mul(ex, rx);
+    mul(ey, ry);
+    mul(ew, rw);
+    for (int i = 0; i < 3; ++i) {
+        ctx[i] = (float)rx[i];
+        ctx[3 + i] = (float)ry[i];
+        ctx[6 + i] = (float)(rw[i] - rp[i]);
+        ctx[9 + i] = (float)rp[i];
+    }
+    ctx[12] = (float)(0.0 - rp[2]);
+}
+
 // A frame's arrays come from `A`: the handle's own arena, or a carved one (the
 // allocation-free frame), which must then name the caller's image `out_image` / `out_pitch`.
 int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_image = nullptr, int out_pitch = 0) {
@@ -311,6 +369,16 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
         for (int j = 0; j < 4; ++j) f.m[4 * i + j] = h->m[4 * j + i];
     f.srgb = h->srgb; f.ms = h->ms;
     f.keep_masks = (h->ms || h->debug) ? 1 : 0;
+    f.has_grad = (h->has_grad && !h->nscenes) ? 1 : 0;
+    if (f.has_grad) {
+        f.grad.table = P_<float4>(h->gtable);
+        f.grad.ramp = P_<uint32_t>(h->gramp);
+        f.grad.nslots = h->gslots;
+        f.grad.rows = (int)h->gramp_rows;
+        f.grad.irh = h->gramp_rows ? 1.f / (float)h->gramp_rows : 0.f;
+        f.grad.xoff = h->ms ? 0.f : 1.f;
+        inverse_projection(h->m, h->W, h->H, f.grad.ctx);
+    }
     f.pitch = carved ? out_pitch : h->W * 4;
     f.R = h->nscenes ? h->batch_rows : (int64_t)(h->H + 1) / 2;
     f.nscenes = h->nscenes;
@@ -623,7 +691,8 @@ int sl_destroy(sl_handle *h) {
                       &h->A.b_rec_mask, &h->A.b_scratch, &h->A.b_items, &h->A.b_imask, &h->A.b_icov, &h->A.b_bicnt, &h->A.b_bmcnt, &h->A.b_bext,
                       &h->A.b_big_list, &h->A.b_small, &h->A.b_nz_flag, &h->A.b_nz_off, &h->A.b_nzk0, &h->A.b_nzk1, &h->A.b_nzv0,
                       &h->A.b_nzv1, &h->A.b_rs_hist, &h->A.b_rs_hoff, &h->A.b_rowbeg, &h->A.b_rowend, &h->A.b_image, &h->A.b_scan_tmp,
-                      &h->A.b_rrec, &h->A.b_btile, &h->A.b_rtile, &h->A.b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry};
+                      &h->A.b_rrec, &h->A.b_btile, &h->A.b_rtile, &h->A.b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry,
+                      &h->gramp, &h->gtable};
     for (DevBuf *b : bufs) release(*b);
     if (h->stage) (void)hipHostFree(h->stage);
     if (h->events_ready) {
@@ -686,6 +755,8 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
         (P && (!s->fill_rule || !s->fill_color)))
         return fail(h, SL_E_ARG, "scene array pointer is NULL");
     if (P == 0 && C != 0) return fail(h, SL_E_ARG, "curves without paths");
+    if ((s->gradient_ramp_size && !s->gradient_ramp) || (s->gradient_table_size && !s->gradient_table))
+        return fail(h, SL_E_ARG, "gradient texture pointer is NULL");
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
     int src = settle(h);
     if (src) return src;
@@ -836,10 +907,17 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
     for_chunks(V, [&](unsigned, size_t lo, size_t hi) {
         for (size_t v = lo; v < hi; ++v) owner[v] = owner[v] == 0xFFFFFFFFu ? 0u : s->curve_path_id[owner[v]];
     });
+    // fill words (ras_base.cpp:144-166): a colour keeps its value unless its alpha is 0; with
+    // a gradient table, a value in [1, 0x01000000] names gradient slot value - 1, which the
+    // compositor evaluates per pixel (output_32.vert.glsl:55-97)
     uint32_t *finfo = (uint32_t *)(st + o_finfo);
+    const uint32_t gslots = s->gradient_table_size / 12;
+    bool has_grad = false;
     for (uint32_t p = 0; p < P; ++p) {
         const uint32_t c = s->fill_color[p];
-        finfo[p] = (c & 0xFF000000u) ? c : 0u;  // ras_base.cpp:147-151
+        const bool grad = !(c & 0xFF000000u) && gslots && c != 0u;
+        finfo[p] = ((c & 0xFF000000u) || grad) ? c : 0u;
+        has_grad |= grad;
     }
     if (C) {
         std::memcpy(st + o_cvpos, s->curve_vertex_pos, 4 * (size_t)C);
@@ -861,6 +939,16 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
     for (const Part &q : parts)
         if (q.bytes) HIPCHK(h, hipMemcpyAsync(q.b->p, st + q.off, q.bytes, hipMemcpyHostToDevice, nullptr));
     HIPCHK(h, hipStreamSynchronize(nullptr));
+    // gradient textures (ras_base.cpp:339-371): the ramp has size / 1024 whole rows
+    const uint32_t rows = s->gradient_ramp_size >> 10;
+    h->gslots = has_grad ? gslots : 0;
+    h->gramp_rows = has_grad ? rows : 0;
+    h->has_grad = has_grad;
+    if (has_grad) {
+        int rc = upload(h, h->gtable, s->gradient_table, (size_t)gslots * 48);
+        if (!rc && rows) rc = upload(h, h->gramp, s->gradient_ramp, (size_t)rows * 4096);
+        if (rc) return rc;
+    }
     h->P = P; h->C = C; h->V = (P == 0) ? 0 : V;
     h->has_scene = true;
     return ensure_tables(h);
@@ -1250,6 +1338,16 @@ int sl_batch_create(int device, const sl_scene *scenes, const int *widths, const
             const int W = widths[ids[k]], H = heights[ids[k]];
             if (sc.n_paths && (!sc.vertex || !sc.curve_vertex_pos || !sc.curve_type || !sc.curve_path_id ||
                                !sc.fill_rule || !sc.fill_color)) { rc = SL_E_ARG; break; }
+            if (sc.gradient_table_size >= 12) {
+                bool grad = false;
+                for (uint32_t p = 0; p < sc.n_paths; ++p)
+                    grad |= !(sc.fill_color[p] & 0xFF000000u) && sc.fill_color[p] != 0u;
+                if (grad) {
+                    b->err = "gradient paints are not supported in batches (scene " + std::to_string(ids[k]) + ")";
+                    rc = SL_E_UNSUPPORTED;
+                    break;
+                }
+            }
             std::memcpy(&vtx[2 * v0], sc.vertex, (size_t)sc.n_vertices * 8);
             for (uint32_t c = 0; c < sc.n_curves; ++c) {
                 cvpos[c0 + c] = sc.curve_vertex_pos[c] + (uint32_t)v0;

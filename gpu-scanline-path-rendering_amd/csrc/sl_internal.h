@@ -58,6 +58,22 @@ struct Tables {
     const float4 *step_c;    // [256] sRGB steps {thr[k+1], dec[k], dec[k+1], 0}
 };
 
+// Gradient paint of a frame (output_32.vert.glsl:55-97, output_32.frag.glsl:45-51).  A
+// fill word g in [1, 0x01000000] names table slot g - 1; the slot paints when its ramp
+// scale (word 7) is positive, otherwise the path draws nothing.
+struct GradDev {
+    const float4 *table;     // [3 * nslots] per slot {tr0, tr1, tr2, tr3}, {tr4, tr5, ramp
+                             //   position (int bits), ramp scale}, {focal x, y, z, 0}
+    const uint32_t *ramp;    // [rows * 1024] RGBA8 ramp texels
+    uint32_t nslots;
+    int rows;                // ramp rows (0: no ramp texture: the paint reads (0, 0, 0, 1))
+    float irh;               // 1 / rows (0 without rows)
+    float xoff;              // x offset of the line ends: 1 in coverage output, 0 in
+                             //   multisample output (output_32.vert.glsl:68, ms_output_32:69)
+    float ctx[13];           // inverse projection rx, ry, rw, rp (x, y, z each), a
+                             //   (vg_main.cpp:117-143)
+};
+
 struct Frame {
     int W, H;
     float m[16];             // rows of the input transform, row-major
@@ -130,6 +146,8 @@ struct Frame {
     int pitch;
     int srgb, ms;
     int keep_masks;          // resolve stores the merged masks (multisample output, debug dumps)
+    int has_grad;            // some fill word names a gradient: the compositor's gradient variant
+    GradDev grad;
     // scan scratch
     uint32_t *scan_tmp;
     size_t scan_tmp_elems;

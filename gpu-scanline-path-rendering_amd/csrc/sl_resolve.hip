@@ -943,6 +943,76 @@ __global__ void k_row_bounds(const uint32_t *__restrict__ keys, const uint32_t *
 }
 
 // ---------------------------------------------------------------------------------
+// gradient paint (output_32.vert.glsl:55-97, output_32.frag.glsl:45-51; the multisample
+// shaders differ only in the x offset of the line ends).  The reference evaluates the
+// gradient coordinates at the two ends of each item's line, (x0 + off, y + 1/2) and
+// (x0 + w + off, y + 1/2) per pixel row, the rasterizer interpolates them linearly to the
+// pixel centre (t = (x - x0 + 1/2) / w), and the fragment shader maps them to the ramp
+// parameter c = min(|d|^2 / (sqrt(B^2 + |d|^2 f.z) - B), 1), d = g - f, B = d . f, and
+// samples the 1024-wide ramp with linear filtering and clamped edges, sRGB-decoding the
+// texels when sRGB correction is on.  The same float operations, in the same order, are
+// restated in the oracle.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ float2 grad_coord(const GradDev &G, const float4 &w0, const float4 &w1, float sx, float sy) {
+    const float *k = G.ctx;
+    const float rdx = (k[0] * sx + k[3] * sy) + k[6];
+    const float rdy = (k[1] * sx + k[4] * sy) + k[7];
+    const float rdz = (k[2] * sx + k[5] * sy) + k[8];
+    const float q = k[12] / rdz;
+    const float ox = k[9] + q * rdx, oy = k[10] + q * rdy;
+    return make_float2((ox * w0.x + oy * w0.y) + w0.z, (ox * w0.w + oy * w1.x) + w1.y);
+}
+
+// A fill word without alpha names gradient slot c - 1; it paints when the slot's ramp scale
+// is positive (the shaders' gradient branch), and draws nothing otherwise.
+__device__ __forceinline__ bool grad_paints(const GradDev &G, uint32_t c) {
+    const uint32_t gi = c - 1u;
+    return gi < G.nslots && G.table[3 * gi + 1].w > 0.f;
+}
+
+__device__ __forceinline__ float4 ramp_texel(const GradDev &G, int i, int j, const float *dc, const float *da) {
+    const uint32_t t = G.ramp[(size_t)j * 1024 + i];
+    return make_float4(dc[t & 0xFFu], dc[(t >> 8) & 0xFFu], dc[(t >> 16) & 0xFFu], da[t >> 24]);
+}
+
+// Paint of gradient word c at pixel (x, yrow) of an item covering columns [x0, x0 + w) of
+// the cell row holding yrow.  dc / da decode a texel's colour and alpha codes.
+__device__ float4 grad_color(const GradDev &G, uint32_t c, int x0, int w, int x, int yrow, const float *dc,
+                             const float *da) {
+    const uint32_t gi = c - 1u;
+    const float4 w0 = G.table[3 * gi], w1 = G.table[3 * gi + 1], w2 = G.table[3 * gi + 2];
+    const float sy = (float)(yrow & ~1) + ((yrow & 1) ? 1.5f : 0.5f);
+    const float2 ga = grad_coord(G, w0, w1, (float)x0 + G.xoff, sy);
+    const float2 gb = grad_coord(G, w0, w1, (float)(x0 + w) + G.xoff, sy);
+    const float t = ((float)(x - x0) + 0.5f) / (float)w;
+    const float omt = 1.f - t;
+    const float gx = ga.x * omt + gb.x * t, gy = ga.y * omt + gb.y * t;
+    const float dx = gx - w2.x, dy = gy - w2.y;
+    const float A = dx * dx + dy * dy, B = dx * w2.x + dy * w2.y;
+    const float den = sqrtf(B * B + A * w2.z) - B;
+    float cc = A * (den > 1e-6f ? 1.f / den : 0.f);
+    cc = cc < 1.f ? cc : 1.f;
+    if (G.rows <= 0) return make_float4(0.f, 0.f, 0.f, 1.f);
+    const int ri = __float_as_int(w1.z);
+    const float u = ((float)(ri & 1023) + 0.5f) * (1.f / 1024.f) + cc * w1.w;
+    const float v = ((float)(ri >> 10) + 0.5f) * G.irh;
+    const float tu = u * 1024.f - 0.5f, tv = v * (float)G.rows - 0.5f;
+    const float fu = floorf(tu), fv = floorf(tv);
+    const float au = tu - fu, av = tv - fv, bu = 1.f - au, bv = 1.f - av;
+    const int iu = (int)fu, jv = (int)fv;
+    const int i0 = tmin(tmax(iu, 0), 1023), i1 = tmin(tmax(iu + 1, 0), 1023);
+    const int j0 = tmin(tmax(jv, 0), G.rows - 1), j1 = tmin(tmax(jv + 1, 0), G.rows - 1);
+    const float4 t00 = ramp_texel(G, i0, j0, dc, da), t10 = ramp_texel(G, i1, j0, dc, da);
+    const float4 t01 = ramp_texel(G, i0, j1, dc, da), t11 = ramp_texel(G, i1, j1, dc, da);
+    float4 o;
+    o.x = (t00.x * bu + t10.x * au) * bv + (t01.x * bu + t11.x * au) * av;
+    o.y = (t00.y * bu + t10.y * au) * bv + (t01.y * bu + t11.y * au) * av;
+    o.z = (t00.z * bu + t10.z * au) * bv + (t01.z * bu + t11.z * au) * av;
+    o.w = (t00.w * bu + t10.w * au) * bv + (t01.w * bu + t11.w * au) * av;
+    return o;
+}
+
+// ---------------------------------------------------------------------------------
 // compositor: one wavefront per 64x2 pixel tile; lane = pixel column, two rows.  For each
 // bucket of the tile's cell row (path order), the lanes blend the item covering their
 // pixels: src-over into an 8-bit framebuffer re-quantised after each blend, coverage
@@ -958,6 +1028,7 @@ struct CompArgs {
     const uint8_t *lut_c, *lut_a;
     uint8_t *image;
     int pitch;
+    GradDev G;
 };
 
 // thr_* hold 257 entries: thr[256] = +inf ends the threshold bump at code 255.
@@ -1004,27 +1075,44 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// Blend one bucket's coverage of this lane's two pixels.  `bits0/bits1` are the sample
-// masks (multisample mode) and `n0/n1` the sample counts (coverage mode).
+// Blend one bucket's coverage of this lane's two pixels with paints p0 / p1 (decoded
+// colour and alpha).  `bits0/bits1` are the sample masks (multisample mode) and `n0/n1`
+// the sample counts (coverage mode).
 template <int SPP, bool MS>
-__device__ __forceinline__ void blend_cov(const LdsTables &T, uint32_t *st0, uint32_t *st1, uint32_t c, uint32_t n0,
-                                          uint32_t n1, uint32_t bits0, uint32_t bits1) {
+__device__ __forceinline__ void blend_cov(const LdsTables &T, uint32_t *st0, uint32_t *st1, const float4 &p0,
+                                          const float4 &p1, uint32_t n0, uint32_t n1, uint32_t bits0, uint32_t bits1) {
     constexpr int NS = MS ? SPP : 1;
-    float sc[3];
-    sc[0] = T.dec_c[c & 0xFFu];
-    sc[1] = T.dec_c[(c >> 8) & 0xFFu];
-    sc[2] = T.dec_c[(c >> 16) & 0xFFu];
-    const float sa = T.dec_a[c >> 24];
+    const float sc0[3] = {p0.x, p0.y, p0.z}, sc1[3] = {p1.x, p1.y, p1.z};
     if (!MS) {
-        st0[0] = blend_px(T, st0[0], sc, sa * ((float)n0 / (float)SPP));
-        st1[0] = blend_px(T, st1[0], sc, sa * ((float)n1 / (float)SPP));
+        st0[0] = blend_px(T, st0[0], sc0, p0.w * ((float)n0 / (float)SPP));
+        st1[0] = blend_px(T, st1[0], sc1, p1.w * ((float)n1 / (float)SPP));
     } else {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            if ((bits0 >> s) & 1u) st0[s] = blend_px(T, st0[s], sc, sa);
-            if ((bits1 >> s) & 1u) st1[s] = blend_px(T, st1[s], sc, sa);
+            if ((bits0 >> s) & 1u) st0[s] = blend_px(T, st0[s], sc0, p0.w);
+            if ((bits1 >> s) & 1u) st1[s] = blend_px(T, st1[s], sc1, p1.w);
         }
     }
+}
+
+__device__ __forceinline__ float4 solid_paint(const LdsTables &T, uint32_t c) {
+    return make_float4(T.dec_c[c & 0xFFu], T.dec_c[(c >> 8) & 0xFFu], T.dec_c[(c >> 16) & 0xFFu], T.dec_a[c >> 24]);
+}
+
+// The paints of fill word c at this lane's two pixels (column x of the item at px0 of
+// width pw, cell row r); false when the word paints nothing.
+template <bool GRAD>
+__device__ __forceinline__ bool item_paint(const CompArgs &A, const LdsTables &T, uint32_t c, int px0, int pw, int x,
+                                           int r, float4 &p0, float4 &p1) {
+    if (GRAD && !(c & 0xFF000000u)) {
+        if (!grad_paints(A.G, c)) return false;
+        const int w = pw ? pw : 2;
+        p0 = grad_color(A.G, c, px0, w, x, 2 * r, T.dec_c, T.dec_a);
+        p1 = grad_color(A.G, c, px0, w, x, 2 * r + 1, T.dec_c, T.dec_a);
+        return true;
+    }
+    p0 = p1 = solid_paint(T, c);
+    return true;
 }
 
 // Items of one bucket are disjoint and sorted by x: the covering item of column x is the
@@ -1041,9 +1129,9 @@ __device__ __forceinline__ int find_item(uint32_t it, int m, int x) {
     return pos;
 }
 
-template <int SPP, bool MS>
+template <int SPP, bool MS, bool GRAD>
 __device__ __noinline__ void blend_bucket_chunked(const CompArgs &A, const LdsTables &T, uint32_t *st0, uint32_t *st1,
-                                                     uint32_t c, int cnt, uint32_t ib, int tx0, int x) {
+                                                     uint32_t c, int cnt, uint32_t ib, int tx0, int x, int r) {
     constexpr int NW = Words<SPP>::n;
     const int lane = threadIdx.x & 63;
     for (int ic = 0; ic < cnt; ic += 64) {
@@ -1070,7 +1158,8 @@ __device__ __noinline__ void blend_bucket_chunked(const CompArgs &A, const LdsTa
                 b1 = pixel_bits<SPP>(mk, x - px0, 1);
             }
         }
-        blend_cov<SPP, MS>(T, st0, st1, c, n0, n1, b0, b1);
+        float4 p0, p1;
+        if (item_paint<GRAD>(A, T, c, px0, pw, x, r, p0, p1)) blend_cov<SPP, MS>(T, st0, st1, p0, p1, n0, n1, b0, b1);
     }
 }
 
@@ -1082,7 +1171,7 @@ __device__ __noinline__ void blend_bucket_chunked(const CompArgs &A, const LdsTa
 // 8-bit framebuffer re-quantised after every blend: coverage alpha = paint alpha *
 // popcount(mask) / N (output_{8,32}.frag.glsl), or per-sample blending plus a resolve in
 // multisample mode (ms_output_{8,32}.frag.glsl).
-template <int SPP, bool MS>
+template <int SPP, bool MS, bool GRAD>
 __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
     constexpr int NW = Words<SPP>::n;
     constexpr uint32_t FULLB = SPP == 32 ? 0xFFFFFFFFu : ((1u << SPP) - 1u);
@@ -1137,7 +1226,7 @@ __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
                 const int cnt = (int)lcnt[w][g0 + g];
                 const uint32_t ib = lib[w][g0 + g];
                 if (cnt > 64) {
-                    blend_bucket_chunked<SPP, MS>(A, T, st0, st1, c, cnt, ib, tx0, x);
+                    blend_bucket_chunked<SPP, MS, GRAD>(A, T, st0, st1, c, cnt, ib, tx0, x, r);
                     continue;
                 }
                 const uint32_t it = gitem[w][g][lane];
@@ -1160,7 +1249,9 @@ __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
                         b1 = pixel_bits<SPP>(mk, x - px0, 1);
                     }
                 }
-                blend_cov<SPP, MS>(T, st0, st1, c, n0, n1, b0, b1);
+                float4 p0, p1;
+                if (item_paint<GRAD>(A, T, c, px0, pw, x, r, p0, p1))
+                    blend_cov<SPP, MS>(T, st0, st1, p0, p1, n0, n1, b0, b1);
             }
             wave_sync_lds();
         }
@@ -1174,7 +1265,9 @@ __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
         const uint32_t bb = inq ? A.rowvals[q] : 0u;
         const uint32_t ext = inq ? A.bext[bb] : 0xFFFFu;
         const int exlo = (int)(ext & 0xFFFFu), exhi = (int)(ext >> 16);
-        const bool ov = inq && exlo < tx1 && exhi > tx0;
+        const uint32_t fc = inq ? A.finfo[A.binfo[bb].x] : 0u;
+        const bool paints = fc != 0u && (!GRAD || (fc & 0xFF000000u) || grad_paints(A.G, fc));
+        const bool ov = inq && exlo < tx1 && exhi > tx0 && paints;
         const unsigned long long om = __ballot(ov);
         if (!om) continue;
         if (nlist + 64 > kListCap) {
@@ -1183,7 +1276,7 @@ __global__ void __launch_bounds__(256) k_composite(CompArgs A) {
         }
         if (ov) {
             const int slot = nlist + (int)__popcll(om & lt);
-            lcol[w][slot] = A.finfo[A.binfo[bb].x];
+            lcol[w][slot] = fc;
             lcnt[w][slot] = A.bicnt[bb];
             lib[w][slot] = 2u * A.boff[bb];
         }
@@ -1293,10 +1386,12 @@ struct CovArgs {
     const uint64_t *rtile;      // per row-list entry: 64-pixel tiles holding items
     const uint32_t *items, *icov;
     const float *dec_c;         // [256] colour decode, for the paint colours
+    const float *dec_a;         // [256] alpha decode (gradient ramp texels)
     const float4 *step_c;       // [256] sRGB steps {thr[k+1], dec[k], dec[k+1], 0}
     const uint8_t *lut_c;       // [4096] sRGB code of i/4096
     uint8_t *image;
     int pitch;
+    GradDev G;
     // batched scenes: per scene {W, H, first tile, first cell row} and image byte offset
     int nscenes;
     int64_t ntiles;
@@ -1330,17 +1425,19 @@ __device__ __noinline__ void scatter_big(const uint32_t *items, uint32_t ib, int
 // (registers capped at 64, a few spilled) hide the latency best.  Groups of 8 buckets
 // need 8 KiB more LDS a workgroup (the sRGB tables included, about 23 KiB), which holds
 // them to four waves per SIMD, so they get the registers of that occupancy.
-template <int SPP, bool SRGB, int kCovGroup, bool BATCH>
+template <int SPP, bool SRGB, int kCovGroup, bool BATCH, bool GRAD>
 __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_composite_cov(CovArgs A) {
     __shared__ float4 s_step[SRGB ? 256 : 1];
     __shared__ uint8_t s_lut[SRGB ? 4096 : 4];
     __shared__ float s_dec[256];
+    __shared__ float s_deca[GRAD ? 256 : 1];
     __shared__ float4 l_src[4][kCovList];
     __shared__ uint2 l_ref[4][kCovList];
     __shared__ uint32_t slot[4][kCovGroup][64];
     for (int i = threadIdx.x; i < 256; i += 256) {
         if (SRGB) s_step[i] = A.step_c[i];
         s_dec[i] = A.dec_c[i];
+        if (GRAD) s_deca[i] = A.dec_a[i];
     }
     for (int i = threadIdx.x; i < (SRGB ? 1024 : 0); i += 256) ((uint32_t *)s_lut)[i] = ((const uint32_t *)A.lut_c)[i];
     __syncthreads();
@@ -1420,7 +1517,7 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_comp
             // {n0, n1, g}; the blend passes then run as many times as the deepest lane's
             // list, each lane blending its own next entry (same per-pixel order as one
             // pass per bucket, fewer passes where the buckets do not overlap).
-            uint32_t ent[kCovGroup];
+            uint32_t ent[kCovGroup], ent2[GRAD ? kCovGroup : 1];
             int nent = 0;
 #pragma unroll
             for (int g = 0; g < kCovGroup; ++g) {
@@ -1445,8 +1542,12 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_comp
                     n1 = (pcv >> (6 * ((x - px0) * 2 + 1))) & 63u;
                 }
                 const uint32_t e = n0 | (n1 << 8) | ((uint32_t)g << 16);
+                const uint32_t e2 = (uint32_t)px0 | ((uint32_t)(pw ? pw : 2) << 16);
 #pragma unroll
-                for (int j = 0; j <= g; ++j) ent[j] = (covered && nent == j) ? e : ent[j];
+                for (int j = 0; j <= g; ++j) {
+                    ent[j] = (covered && nent == j) ? e : ent[j];
+                    if (GRAD) ent2[j] = (covered && nent == j) ? e2 : ent2[j];
+                }
                 nent += covered ? 1 : 0;
             }
 #pragma unroll
@@ -1455,8 +1556,18 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_comp
                 if (nent > j) {
                     const uint32_t e = ent[j];
                     const float4 src = l_src[w][g0 + (int)(e >> 16)];
-                    blend_dec<SRGB>(s_step, s_lut, d0, src, src.w * ((float)(e & 63u) / (float)SPP));
-                    blend_dec<SRGB>(s_step, s_lut, d1, src, src.w * ((float)((e >> 8) & 63u) / (float)SPP));
+                    if (GRAD && src.w < 0.f) {
+                        // a gradient paint: its colour at each of the lane's two pixels
+                        const uint32_t c = __float_as_uint(src.x);
+                        const int ix0 = (int)(ent2[j] & 0xFFFFu), iw = (int)(ent2[j] >> 16);
+                        const float4 p0 = grad_color(A.G, c, ix0, iw, x, 2 * r, s_dec, s_deca);
+                        const float4 p1 = grad_color(A.G, c, ix0, iw, x, 2 * r + 1, s_dec, s_deca);
+                        blend_dec<SRGB>(s_step, s_lut, d0, p0, p0.w * ((float)(e & 63u) / (float)SPP));
+                        blend_dec<SRGB>(s_step, s_lut, d1, p1, p1.w * ((float)((e >> 8) & 63u) / (float)SPP));
+                    } else {
+                        blend_dec<SRGB>(s_step, s_lut, d0, src, src.w * ((float)(e & 63u) / (float)SPP));
+                        blend_dec<SRGB>(s_step, s_lut, d1, src, src.w * ((float)((e >> 8) & 63u) / (float)SPP));
+                    }
                 }
             }
             wave_sync_lds();
@@ -1471,7 +1582,9 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_comp
         const uint4 rec = inq ? A.rrec[qc + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
         const uint64_t rt = inq ? A.rtile[qc + lane] : 0ull;
         const int exlo = (int)(rec.x & 0xFFFFu), exhi = (int)(rec.x >> 16);
-        const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tbit) & 1ull);
+        // a word without alpha paints nothing, unless it names a gradient that paints
+        const bool paints = (rec.y & 0xFF000000u) || (GRAD && rec.y != 0u && grad_paints(A.G, rec.y));
+        const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tbit) & 1ull) && paints;
         const unsigned long long om = __ballot(ov);
         if (!om) continue;
         const int nov = (int)__popcll(om);
@@ -1482,8 +1595,11 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_comp
         if (ov) {
             const int sl = nlist + (int)__popcll(om & lt);
             const uint32_t c = rec.y;
-            l_src[w][sl] = make_float4(s_dec[c & 0xFFu], s_dec[(c >> 8) & 0xFFu], s_dec[(c >> 16) & 0xFFu],
-                                       unorm_dec((int)(c >> 24)));
+            if (GRAD && !(c & 0xFF000000u))
+                l_src[w][sl] = make_float4(__uint_as_float(c), 0.f, 0.f, -1.f);
+            else
+                l_src[w][sl] = make_float4(s_dec[c & 0xFFu], s_dec[(c >> 8) & 0xFFu], s_dec[(c >> 16) & 0xFFu],
+                                           unorm_dec((int)(c >> 24)));
             l_ref[w][sl] = make_uint2(rec.w, rec.z);
         }
         nlist += nov;
@@ -1710,19 +1826,22 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
         CovArgs C;
         C.W = f.W; C.H = f.H; C.tiles_x = tiles_x;
         C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec; C.rtile = f.rtile;
-        C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.step_c = t.step_c; C.lut_c = t.lut_c;
+        C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.dec_a = t.dec_a; C.step_c = t.step_c; C.lut_c = t.lut_c;
         C.image = f.image; C.pitch = f.pitch;
+        C.G = f.grad;
         C.nscenes = f.nscenes; C.ntiles = ntiles; C.stile = f.stile; C.simg = f.simg;
         // one scene: 8 XCD row sets of ceil(R / 8) rows each; a batch: tiles in order
         dim3 g(f.nscenes ? blocks_for(ntiles, 4) : 8u * blocks_for((int64_t)tiles_x * ((R + 7) / 8), 4)), b(256);
 #define SL_LAUNCH(N, SR)                                                                   \
     do {                                                                                   \
         if (f.nscenes) {                                                                   \
-            if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4, true>), g, b, 0, s, C); \
-            else hipLaunchKernelGGL((k_composite_cov<N, SR, 8, true>), g, b, 0, s, C);               \
+            if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4, true, false>), g, b, 0, s, C); \
+            else hipLaunchKernelGGL((k_composite_cov<N, SR, 8, true, false>), g, b, 0, s, C);               \
+        } else if (f.has_grad) {                                                           \
+            hipLaunchKernelGGL((k_composite_cov<N, SR, 4, false, true>), g, b, 0, s, C);   \
         } else {                                                                           \
-            if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4, false>), g, b, 0, s, C); \
-            else hipLaunchKernelGGL((k_composite_cov<N, SR, 8, false>), g, b, 0, s, C);               \
+            if (f.comp_group == 4) hipLaunchKernelGGL((k_composite_cov<N, SR, 4, false, false>), g, b, 0, s, C); \
+            else hipLaunchKernelGGL((k_composite_cov<N, SR, 8, false, false>), g, b, 0, s, C);               \
         }                                                                                  \
     } while (0)
         if (f.srgb) {
@@ -1744,10 +1863,17 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
     A.items = f.items; A.imask = f.imask; A.icov = f.icov;
     A.dec_c = t.dec_c; A.thr_c = t.thr_c; A.dec_a = t.dec_a; A.thr_a = t.thr_a; A.lut_c = t.lut_c; A.lut_a = t.lut_a;
     A.image = f.image; A.pitch = f.pitch;
+    A.G = f.grad;
     dim3 g(blocks_for(ntiles, 4)), b(256);
-    if (t.spp == 8) hipLaunchKernelGGL((k_composite<8, true>), g, b, 0, s, A);
-    else if (t.spp == 16) hipLaunchKernelGGL((k_composite<16, true>), g, b, 0, s, A);
-    else hipLaunchKernelGGL((k_composite<32, true>), g, b, 0, s, A);
+    if (f.has_grad) {
+        if (t.spp == 8) hipLaunchKernelGGL((k_composite<8, true, true>), g, b, 0, s, A);
+        else if (t.spp == 16) hipLaunchKernelGGL((k_composite<16, true, true>), g, b, 0, s, A);
+        else hipLaunchKernelGGL((k_composite<32, true, true>), g, b, 0, s, A);
+    } else {
+        if (t.spp == 8) hipLaunchKernelGGL((k_composite<8, true, false>), g, b, 0, s, A);
+        else if (t.spp == 16) hipLaunchKernelGGL((k_composite<16, true, false>), g, b, 0, s, A);
+        else hipLaunchKernelGGL((k_composite<32, true, false>), g, b, 0, s, A);
+    }
 }
 
 void launch_encode_values(const Tables &t, int channel, const float *v, int64_t n, uint8_t *code, float *dec,

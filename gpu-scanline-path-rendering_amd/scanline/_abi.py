@@ -41,6 +41,10 @@ class SlScene(ctypes.Structure):
         ("curve_arc_w", ctypes.c_void_p),
         ("fill_rule", ctypes.c_void_p),
         ("fill_color", ctypes.c_void_p),
+        ("gradient_ramp", ctypes.c_void_p),
+        ("gradient_ramp_size", ctypes.c_uint32),
+        ("gradient_table", ctypes.c_void_p),
+        ("gradient_table_size", ctypes.c_uint32),
     ]
 
 

@@ -1,6 +1,6 @@
 """RVG scene files (the reference's text scene format) into a VGContainer.
 
-Follows `Mochimazui::RVG::load` (gpu-scanline/src/rvg.cpp:62-520) for solid paints:
+Follows `Mochimazui::RVG::load` (gpu-scanline/src/rvg.cpp:62-758):
 
 * the file opens with viewport and window corners (rvg.cpp:154-162), then `scene
   dyn_identity`;
@@ -20,15 +20,24 @@ Follows `Mochimazui::RVG::load` (gpu-scanline/src/rvg.cpp:62-520) for solid pain
   only the first len - 6 characters after `rgb(`, dropping the last digit of blue; that
   is kept here too, so the same file gives the same colours.  `a128` requantises alpha
   to 7 bits as the reference's option does;
-* comment tokens `//` skip their line.
-
-Gradient paints (`linear`, `radial`) are not supported by this renderer: `gradients`
-chooses between raising and skipping those paths.
+* comment tokens `//` skip their line;
+* gradient paints, `linear` / `radial` followed by the rest of their line: every
+  character that cannot belong to a number becomes a blank, the numbers before `pad` are
+  x0 y0 x1 y1 [r] and the stops (offset r g b a), the six after it the paint transform
+  (rvg.cpp:490-520, 547-580).  The path's fill becomes the gradient (fill word path + 1,
+  colour (255, 0, 255, 0)), and the file's gradient table and ramp textures are built as
+  the reference builds them (`gradient.GradientTextures`, with the ramp in sRGB when
+  `srgb`, the reference's `--srgb` option, is on).  Where the reference would read a
+  number that is not there, this loader raises.
 """
 
 import numpy as np
 
+import re
+
+from .gradient import GradientTextures
 from .scene import FT_COLOR, FT_GRADIENT, FT_NONE, VGContainer, rgba
+from .vgfile import decimal_to_f32
 
 _F32 = np.float32
 
@@ -93,8 +102,45 @@ def _affine(tok):
     return [_F32(v) for v in vals]
 
 
-def loads(text, a128=False, gradients="raise"):
-    """Parse RVG text; returns (VGContainer, viewport corners, window corners)."""
+_NUM = re.compile(r"[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?")
+
+
+def _gradient_numbers(rest):
+    """The two number streams of a gradient paint's line: (before `pad`, after it), with
+    every character that cannot belong to a number blanked first (rvg.cpp:498-509)."""
+    ppad = rest.find("pad")
+    if ppad < 0:
+        raise RVGError("gradient paint without 'pad'")
+    chars = list(rest)
+    n = len(rest)
+    for i, ch in enumerate(rest):
+        keep = ("0" <= ch <= "9") or ch in "-.+" or (ch == "e" and i < n - 1 and rest[i + 1] in "+-")
+        if not keep:
+            chars[i] = " "
+    s = "".join(chars)
+
+    def numbers(t):
+        out, i = [], 0
+        while True:
+            while i < len(t) and t[i] == " ":
+                i += 1
+            if i >= len(t):
+                return out
+            m = _NUM.match(t, i)
+            if not m:
+                return out  # the stream fails here: later reads get nothing
+            out.append(m.group(0))
+            i = m.end()
+
+    return numbers(s[:ppad]), numbers(s[ppad:])
+
+
+def loads(text, a128=False, gradients="render", srgb=True):
+    """Parse RVG text; returns (VGContainer, viewport corners, window corners).
+
+    gradients: "render" builds the gradient textures; "skip" leaves gradient-painted paths
+    unpainted; "raise" refuses them.  srgb: the ramps are built for sRGB correction (the
+    reference's --srgb), which must match the renderer's setting."""
     toks = _tokens(text)
     n = len(toks)
     pos = 0
@@ -125,6 +171,8 @@ def loads(text, a128=False, gradients="raise"):
     last_path_vertices = -1
     path = -1
     verts = vg._v  # vertex list of the container, appended in place below
+    gtex = GradientTextures(srgb)
+    n_grad = 0
     while pos < n:
         tok = nxt()
         if tok == "//":
@@ -226,6 +274,7 @@ def loads(text, a128=False, gradients="raise"):
         opacity = _F32(float(nxt()))
         kind = nxt()
         if kind == "solid":
+            gtex.add_solid()
             r, g, b, al = _colour(nxt())
             cr, cg, cb, ca = _u8(r), _u8(g), _u8(b), _u8(al)
             ca = int(np.trunc(_F32(ca) * opacity)) & 0xFF
@@ -237,10 +286,33 @@ def loads(text, a128=False, gradients="raise"):
         elif kind in ("linear", "radial"):
             if gradients == "raise":
                 raise RVGError("gradient paints are not supported")
-            vg._ftype[path] = FT_GRADIENT  # not rendered here: the path draws nothing
-            skip_line()
+            ln = toks[pos - 1][1]
+            rest = []
+            while pos < n and toks[pos][1] == ln:
+                rest.append(toks[pos][0])
+                pos += 1
+            head, tail = _gradient_numbers(" " + " ".join(rest))
+            ngeo = 5 if kind == "radial" else 4
+            if len(head) < ngeo or len(tail) < 6 or (len(head) - ngeo) % 5:
+                raise RVGError("gradient paint with missing numbers: %r" % " ".join(rest))
+            h = decimal_to_f32(head) if head else []
+            geo = [float(v) for v in h[:ngeo]]
+            x0, y0, x1, y1 = geo[:4]
+            rad = geo[4] if kind == "radial" else 0.0
+            stops = [tuple(float(v) for v in h[k:k + 5]) for k in range(ngeo, len(h), 5)]
+            tr = [float(v) for v in decimal_to_f32(tail[:6])]
+            gtex.add(kind, x0, y0, x1, y1, rad, stops, opacity, tr)
+            vg._ftype[path] = FT_GRADIENT
+            vg._color[path] = rgba(255, 0, 255, 0)
+            vg._findex[path] = n_grad
+            n_grad += 1
+            if gradients == "skip":
+                vg._ftype[path] = FT_NONE
         else:
             raise RVGError("unknown paint %r" % kind)
+    ramp, table = gtex.textures()
+    vg.gradient_ramp = ramp.astype(np.int64).tolist()
+    vg.gradient_table = table.astype(np.int64).tolist()
     return vg, viewport, window
 
 

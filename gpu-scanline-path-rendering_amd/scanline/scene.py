@@ -16,7 +16,9 @@ class SceneArrays:
     """Owns numpy arrays laid out as include/scanline.h's sl_scene."""
 
     def __init__(self, vertex, curve_vertex_pos, curve_type, curve_path_id, curve_arc_w,
-                 fill_rule, fill_color):
+                 fill_rule, fill_color, gradient_ramp=None, gradient_table=None):
+        """fill_color holds the fill word of each path (ras_base.cpp:144-166): an RGBA8
+        colour, or path + 1 for a gradient paint found in the gradient textures."""
         self.vertex = np.ascontiguousarray(vertex, dtype=np.float32).reshape(-1, 2)
         self.curve_vertex_pos = np.ascontiguousarray(curve_vertex_pos, dtype=np.uint32)
         self.curve_type = np.ascontiguousarray(curve_type, dtype=np.uint8)
@@ -24,6 +26,14 @@ class SceneArrays:
         self.curve_arc_w = np.ascontiguousarray(curve_arc_w, dtype=np.float32)
         self.fill_rule = np.ascontiguousarray(fill_rule, dtype=np.uint8)
         self.fill_color = np.ascontiguousarray(fill_color, dtype=np.uint32)
+        # gradient textures (vg_container.h:344-345): RGBA8 ramp texels, 1024 a row, and
+        # 12 words of gradient table per path slot
+        self.gradient_ramp = np.ascontiguousarray(
+            np.zeros(0) if gradient_ramp is None else np.asarray(gradient_ramp, dtype=np.int64) & 0xFFFFFFFF,
+            dtype=np.uint32).view(np.int32)
+        self.gradient_table = np.ascontiguousarray(
+            np.zeros(0) if gradient_table is None else np.asarray(gradient_table, dtype=np.int64) & 0xFFFFFFFF,
+            dtype=np.uint32).view(np.int32)
 
     @property
     def n_paths(self):
@@ -48,13 +58,17 @@ class SceneArrays:
         s.curve_arc_w = self.curve_arc_w.ctypes.data
         s.fill_rule = self.fill_rule.ctypes.data
         s.fill_color = self.fill_color.ctypes.data
+        s.gradient_ramp = self.gradient_ramp.ctypes.data if self.gradient_ramp.size else None
+        s.gradient_ramp_size = self.gradient_ramp.size
+        s.gradient_table = self.gradient_table.ctypes.data if self.gradient_table.size else None
+        s.gradient_table_size = self.gradient_table.size
         return s
 
     def oracle_args(self):
         """Positional arrays in the order of the oracle's oracle_run."""
         return (self.n_paths, self.n_curves, self.n_vertices, self.vertex, self.curve_vertex_pos,
                 self.curve_type, self.curve_path_id, self.curve_arc_w, self.fill_rule,
-                self.fill_color)
+                self.fill_color, self.gradient_ramp, self.gradient_table)
 
 
 # fill and stroke types of a path (VGFillType / VGStrokeType, vg_container.h:19-29)
@@ -142,13 +156,19 @@ class VGContainer:
 
     def fillInfo(self):
         """Per-path fill word as the rasterizer's input gathers it (ras_base.cpp:144-166):
-        colour fills keep their colour unless its alpha is 0, every other fill type gives 0
-        (gradient paints are not rendered here, so such paths draw nothing); a path with a
-        colour stroke is filled with the stroke colour instead, since the reference's
-        renderer takes its input without stroke-to-fill conversion."""
+        colour fills keep their colour unless its alpha is 0, a gradient fill gives the
+        path's index + 1 (the renderer finds its gradient in the gradient table), every
+        other fill type gives 0; a path with a colour stroke is filled with the stroke colour
+        instead, since the reference's renderer takes its input without stroke-to-fill
+        conversion."""
         out = []
-        for c, t, sc, st in zip(self._color, self._ftype, self._scolor, self._stype):
-            w = c if t == FT_COLOR and (c & 0xFF000000) else 0
+        for p, (c, t, sc, st) in enumerate(zip(self._color, self._ftype, self._scolor, self._stype)):
+            if t == FT_COLOR:
+                w = c if (c & 0xFF000000) else 0
+            elif t == FT_GRADIENT:
+                w = p + 1
+            else:
+                w = 0
             if st == ST_COLOR:
                 w = sc if sc & 0xFF000000 else 0
             out.append(w)
@@ -157,7 +177,7 @@ class VGContainer:
     def arrays(self):
         v = np.array(self._v, dtype=np.float32).reshape(-1, 2)
         return SceneArrays(v, self._cpos, self._ctype, self._cpath, self._cw, self._rule,
-                           self.fillInfo())
+                           self.fillInfo(), self.gradient_ramp, self.gradient_table)
 
     @classmethod
     def from_arrays(cls, a, width=0, height=0):
@@ -188,7 +208,7 @@ class VGContainer:
         vgfile.save(self, filename)
 
     @staticmethod
-    def load(filename, gradients="raise"):
+    def load(filename, gradients="render"):
         """VGContainer::load (vg_container.cpp:1679-1689) as a constructor: `.vgt` or `.vgb`."""
         from . import vgfile
         return vgfile.load(filename, gradients=gradients)

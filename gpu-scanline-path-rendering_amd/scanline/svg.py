@@ -37,9 +37,18 @@ values (`rgba(...)` and percentage colours, `circle` / `ellipse` without x / y /
 height, a `color` attribute, which pops the reference's attribute stack, unparseable
 numbers) raise `SVGError` with no partial result.
 
-Colour strokes fill the path with the stroke colour (see `VGContainer.fillInfo`);
-gradient fills are not rendered by this renderer: `gradients="raise"` (default) refuses
-them, `"skip"` keeps them as unrendered paths.
+Colour strokes fill the path with the stroke colour (see `VGContainer.fillInfo`).
+
+Gradient fills.  The reference's SVG loader keeps the gradient definitions but builds no
+gradient textures (only its RVG loader does, rvg.cpp:490-758), so its renderer reads an
+empty gradient table for such paths.  With `gradients="render"` (the default) this loader
+builds the textures with the RVG loader's construction (`gradient.GradientTextures`): a
+`linearGradient` from (x1, y1) to (x2, y2), a `radialGradient` of centre (cx, cy), focus
+(fx, fy) (the centre when absent) and radius r, stops of the stop colours and opacities, the
+path's fill-opacity, and a paint transform that maps gradient coordinates to the scene's
+final coordinates: the gradientTransform, then the element's transform, then the viewBox
+shift and y flip.  `"skip"` leaves gradient fills without textures (they draw nothing) and
+`"raise"` refuses them.  `srgb` selects the ramp encoding (the renderer's sRGB setting).
 """
 import math
 import re
@@ -47,6 +56,7 @@ import xml.parsers.expat
 
 import numpy as np
 
+from .gradient import GradientTextures
 from .scene import FT_COLOR, FT_GRADIENT, FT_NONE, ST_COLOR, ST_GRADIENT, ST_NONE, VGContainer, rgba
 from .vgfile import decimal_to_f32
 
@@ -444,6 +454,7 @@ class _Loader:
         self.vg = VGContainer()
         self.stack = []
         self.gmap = {}  # gradient id -> index
+        self.pmat = {}  # path -> (element transform, fill opacity) when it ended
         self.vg.svg_gradients = []
         self.gradient = None
         self.stop = dict(color=(0, 0, 0, 255), opacity=F(1), offset=F(0))
@@ -553,6 +564,7 @@ class _Loader:
         vg._stype[pi] = a["stroke"]
         vg._scolor[pi] = rgba(*a["scolor"])
         m = a["mat"]
+        self.pmat[pi] = (m, a["fopacity"])
         for k in range(vbegin, vg.vertexNumber()):
             x, y = _apply(m, *vg._v[k])
             vg._v[k] = (float(x), float(y))
@@ -706,18 +718,53 @@ class _Loader:
         vg._v = [(float(F(x) - vx), float(hf - (F(y) - vy))) for x, y in vg._v]
 
 
-def loads(text, a128=False, gradients="raise", on_error="raise"):
+def _affine(m):
+    """Rows (a, b, c), (d, e, f) of a 3x3 column-major affine matrix, in float64."""
+    return np.array([[m[0][0], m[1][0], m[2][0]], [m[0][1], m[1][1], m[2][1]], [0.0, 0.0, 1.0]], dtype=np.float64)
+
+
+def _gradient_textures(L, srgb):
+    """Gradient table and ramp for the loaded paths (see the module notes)."""
+    vg = L.vg
+    g = GradientTextures(srgb)
+    vx, vy = float(L.viewbox[0]), float(L.viewbox[1])
+    hf = float(F(vg.height & 0xFFFFFFFF))
+    final = np.array([[1.0, 0.0, -vx], [0.0, -1.0, hf + vy], [0.0, 0.0, 1.0]])
+    for pi in range(vg.pathNumber()):
+        if vg._ftype[pi] != FT_GRADIENT or vg._stype[pi] == ST_COLOR or pi not in L.pmat:
+            g.add_solid()
+            continue
+        gd = vg.svg_gradients[vg._findex[pi]]
+        m, opacity = L.pmat[pi]
+        tr = final @ _affine(m) @ _affine(gd["transform"])
+        stops = [(float(st["offset"]),) + tuple(c / 255.0 for c in st["color"]) for st in gd["stops"]]
+        words = [tr[0, 0], tr[0, 1], tr[0, 2], tr[1, 0], tr[1, 1], tr[1, 2]]
+        val = lambda k, d=0.0: float(gd.get(k, d))  # noqa: E731
+        if gd["type"] == "linear":
+            g.add("linear", val("x1"), val("y1"), val("x2"), val("y2"), 0.0, stops, float(opacity), words)
+        else:
+            cx, cy = val("cx"), val("cy")
+            g.add("radial", cx, cy, val("fx", cx), val("fy", cy), val("r"), stops, float(opacity), words)
+    ramp, table = g.textures()
+    vg.gradient_ramp = ramp.astype(np.int64).tolist()
+    vg.gradient_table = table.astype(np.int64).tolist()
+
+
+def loads(text, a128=False, gradients="render", on_error="raise", srgb=True):
     """Parse SVG text into a VGContainer (see the module notes)."""
-    if gradients not in ("raise", "skip") or on_error not in ("raise", "partial"):
-        raise ValueError("gradients: 'raise' or 'skip'; on_error: 'raise' or 'partial'")
+    if gradients not in ("render", "raise", "skip") or on_error not in ("raise", "partial"):
+        raise ValueError("gradients: 'render', 'raise' or 'skip'; on_error: 'raise' or 'partial'")
     L = _Loader(a128, gradients)
     stopped = L.run(_parse_xml(text))
     vg = L.vg
     if stopped is not None and on_error == "raise":
         raise SVGError("the reference's SVG::load stops here: %s" % stopped, partial=vg)
     # a colour stroke replaces the fill word, so only gradient fills without one need a gradient
-    if gradients == "raise" and any(t == FT_GRADIENT and st != ST_COLOR for t, st in zip(vg._ftype, vg._stype)):
-        raise SVGError("gradient paints are not supported")
+    used = any(t == FT_GRADIENT and st != ST_COLOR for t, st in zip(vg._ftype, vg._stype))
+    if gradients == "raise" and used:
+        raise SVGError("gradient paints refused (gradients='raise')")
+    if gradients == "render" and used:
+        _gradient_textures(L, srgb)
     return vg
 
 

@@ -186,7 +186,7 @@ def _parse_binary(data):
     return f
 
 
-def load_binary(filename, gradients="raise"):
+def load_binary(filename, gradients="render"):
     """VGContainer::loadBinary (vg_container.cpp:1852-1906)."""
     with open(filename, "rb") as fh:
         return _container(_parse_binary(fh.read()), gradients)
@@ -336,7 +336,7 @@ def _parse_text(data):
     return f
 
 
-def load_text(filename, gradients="raise"):
+def load_text(filename, gradients="render"):
     """VGContainer::loadTxt (vg_container.cpp:1744-1793)."""
     with open(filename, "rb") as fh:
         return _container(_parse_text(fh.read()), gradients)
@@ -358,11 +358,12 @@ def save(vg, filename):
         raise VGFileError("unknown scene file suffix %r (vgt or vgb)" % sfx)
 
 
-def load(filename, gradients="raise"):
-    """VGContainer::load: the suffix picks the format.  `gradients` ("raise" or "skip")
-    says what to do with gradient-filled paths, which this renderer does not draw."""
-    if gradients not in ("raise", "skip"):
-        raise ValueError("gradients must be 'raise' or 'skip'")
+def load(filename, gradients="render"):
+    """VGContainer::load: the suffix picks the format.  The file's gradient ramp and table
+    textures travel with the container, so gradient-filled paths render ("render" or
+    "skip", which keep them as they are); "raise" refuses them."""
+    if gradients not in ("render", "raise", "skip"):
+        raise ValueError("gradients must be 'render', 'raise' or 'skip'")
     sfx = _suffix(filename)
     if sfx == "vgt":
         return load_text(filename, gradients)

@@ -57,6 +57,15 @@ typedef struct sl_handle sl_handle;
  * (vg_container.cpp:97-125).  Curves must be grouped by path with non-decreasing
  * curve_path_id; every contour must be closed (the reference's loaders close open
  * contours: rvg.cpp:383-396, svg_load.cpp:784-795).
+ *
+ * fill_color is the path's fill word (fill_info, ras_base.cpp:144-166): an RGBA8 colour
+ * (alpha 0 draws nothing), or, when a gradient table is given, a value g in
+ * [1, 0x01000000] naming the gradient in table slot g - 1 (the reference's packing gives a
+ * gradient-filled path p the word p + 1).  The gradient textures are the VGContainer's
+ * (vg_container.h:344-345): gradient_ramp holds RGBA8 texels in rows of 1024, and
+ * gradient_table 12 words per slot {paint transform tr[6] (float), ramp position (int),
+ * ramp scale (float), focal point x, y, z (float), unused}, as RVG::load builds them
+ * (rvg.cpp:490-758).  Sizes count 32-bit words; both may be 0 / NULL.
  */
 typedef struct sl_scene {
     uint32_t        n_paths;
@@ -68,7 +77,12 @@ typedef struct sl_scene {
     const uint32_t *curve_path_id;     /* [n_curves] owning path, non-decreasing      */
     const float    *curve_arc_w;       /* [n_curves] rational weight w1; may be NULL  */
     const uint8_t  *fill_rule;         /* [n_paths]  SL_FILL_*                         */
-    const uint32_t *fill_color;        /* [n_paths]  RGBA8, r | g<<8 | b<<16 | a<<24   */
+    const uint32_t *fill_color;        /* [n_paths]  fill word: RGBA8 r | g<<8 | b<<16 | a<<24,
+                                          or gradient slot + 1                        */
+    const int32_t  *gradient_ramp;     /* [gradient_ramp_size] RGBA8 texels; may be NULL */
+    uint32_t        gradient_ramp_size;
+    const int32_t  *gradient_table;    /* [gradient_table_size] 12 words per slot; may be NULL */
+    uint32_t        gradient_table_size;
 } sl_scene;
 
 /* ElementNumber (ras_base.h:124-131), widened to 64 bits. */

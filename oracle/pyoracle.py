@@ -42,7 +42,7 @@ def lib(omp=False):
     L = ctypes.CDLL(path)
     vp, u32, i = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
     L.oracle_run.restype = vp
-    L.oracle_run.argtypes = [u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, i, i, i, i, i, vp, i]
+    L.oracle_run.argtypes = [u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, i, i, i, i, i, vp, i, vp, u32, vp, u32]
     L.oracle_get.restype = i
     L.oracle_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t)]
     L.oracle_render_seconds.restype = ctypes.c_double
@@ -101,14 +101,16 @@ class OracleResult:
 def run(scene, width, height, samples, srgb=True, ms=False, transform=None, keep=True, omp=False):
     """Render `scene` (scanline.scene.SceneArrays) on the CPU oracle."""
     L = lib(omp)
-    P, C, V, vtx, cvpos, ctype, cpath, cw, rule, color = scene.oracle_args()
+    P, C, V, vtx, cvpos, ctype, cpath, cw, rule, color, ramp, table = scene.oracle_args()
     m = None
     if transform is not None:
         m = np.ascontiguousarray(np.asarray(transform, dtype=np.float32).reshape(16))
     ptr = L.oracle_run(P, C, V, vtx.ctypes.data, cvpos.ctypes.data, ctype.ctypes.data, cpath.ctypes.data,
                        cw.ctypes.data, rule.ctypes.data, color.ctypes.data, int(width), int(height),
                        int(samples), int(bool(srgb)), int(bool(ms)),
-                       m.ctypes.data if m is not None else None, 1 if keep else 0)
+                       m.ctypes.data if m is not None else None, 1 if keep else 0,
+                       ramp.ctypes.data if ramp.size else None, ramp.size,
+                       table.ctypes.data if table.size else None, table.size)
     if not ptr:
         raise ValueError("oracle_run rejected the arguments")
     return OracleResult(L, ptr)

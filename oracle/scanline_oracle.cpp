@@ -377,6 +377,9 @@ struct Scene {
     std::vector<uint8_t> frule;
     std::vector<uint32_t> finfo;
     std::vector<uint32_t> vpath;  // vertex_path_id
+    // gradient textures (ras_base.cpp:339-371): ramp texels in rows of 1024, 12-word slots
+    std::vector<int32_t> ramp, table;
+    uint32_t gslots = 0, rows = 0;
 };
 
 struct Result {
@@ -800,6 +803,126 @@ void inclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------------
+// gradient paint (output_32.vert.glsl:55-97, output_32.frag.glsl:45-51, ms_output_*.glsl)
+// ---------------------------------------------------------------------------------
+// The driver's inverse projection (updateRasterizerTransform, vg_main.cpp:117-143): with R
+// the inverse of the input transform, rp = R (W/2, H/2, -min(W, H), 1), rx / ry = R's first
+// two columns, rw = R's last column - rp, a = -rp.z.  In double (Gauss-Jordan, partial
+// pivoting), rounded to float once; a singular transform gives zeros.
+void inverse_projection(const float *m, int W, int H, float ctx[13]) {
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            a[r][c] = (double)m[4 * c + r];
+            a[r][4 + c] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) {
+            for (int i = 0; i < 13; ++i) ctx[i] = 0.f;
+            return;
+        }
+        if (piv != col)
+            for (int c = 0; c < 8; ++c) std::swap(a[col][c], a[piv][c]);
+        const double d = a[col][col];
+        for (int c = 0; c < 8; ++c) a[col][c] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f != 0.0)
+                for (int c = 0; c < 8; ++c) a[r][c] -= f * a[col][c];
+        }
+    }
+    auto mul = [&](const double v[4], double o[4]) {
+        for (int r = 0; r < 4; ++r) o[r] = a[r][4] * v[0] + a[r][5] * v[1] + a[r][6] * v[2] + a[r][7] * v[3];
+    };
+    const double w = (double)W, h = (double)H;
+    const double vp[4] = {w / 2, h / 2, -std::min(w, h), 1.0}, ex[4] = {1, 0, 0, 0}, ey[4] = {0, 1, 0, 0},
+                 ew[4] = {0, 0, 0, 1};
+    double rp[4], rx[4], ry[4], rw[4];
+    mul(vp, rp);
+    mul(ex, rx);
+    mul(ey, ry);
+    mul(ew, rw);
+    for (int i = 0; i < 3; ++i) {
+        ctx[i] = (float)rx[i];
+        ctx[3 + i] = (float)ry[i];
+        ctx[6 + i] = (float)(rw[i] - rp[i]);
+        ctx[9 + i] = (float)rp[i];
+    }
+    ctx[12] = (float)(0.0 - rp[2]);
+}
+
+// A fill word without alpha names gradient slot c - 1; it paints when the slot's ramp scale
+// (word 7) is positive, as the shaders' gradient branch requires.
+inline bool gradient_paints(const Scene &s, uint32_t c) {
+    const uint32_t gi = c - 1u;
+    return c != 0u && gi < s.gslots && bitsf((uint32_t)s.table[12 * (size_t)gi + 7]) > 0.f;
+}
+
+// Gradient coordinates of window point (sx, sy): obj = rp + a / rd.z * rd with
+// rd = rx sx + ry sy + rw (output_32.vert.glsl:70-76), then the slot's affine map (:79-89).
+inline void gradient_coord(const float *k, const float *w, float sx, float sy, float &gx, float &gy) {
+    const float rdx = (k[0] * sx + k[3] * sy) + k[6];
+    const float rdy = (k[1] * sx + k[4] * sy) + k[7];
+    const float rdz = (k[2] * sx + k[5] * sy) + k[8];
+    const float q = k[12] / rdz;
+    const float ox = k[9] + q * rdx, oy = k[10] + q * rdy;
+    gx = (ox * w[0] + oy * w[1]) + w[2];
+    gy = (ox * w[3] + oy * w[4]) + w[5];
+}
+
+// Paint of gradient word c at pixel (x, y) of the item covering columns [x0, x0 + wd) of y's
+// cell row: the line ends at x0 + xoff and x0 + wd + xoff (xoff 1 in coverage output, 0 in
+// multisample output, output_32.vert.glsl:68 / ms_output_32.vert.glsl:69), interpolated to
+// the pixel centre; ramp parameter c = min(|d|^2 / (sqrt(B^2 + |d|^2 f.z) - B), 1) with
+// d = g - f, B = d . f (safeRcpP: 0 unless the divisor exceeds 1e-6); the ramp texture
+// (1024 wide, `rows` high) read with linear filtering and clamped edges, texels decoded by
+// dc (colour: sRGB or UNORM) and da (alpha); no ramp texture reads (0, 0, 0, 1).
+void gradient_paint(const Scene &s, const float *ctx, float xoff, uint32_t c, int x0, int wd, int x, int y,
+                    const float *dc, const float *da, float out[4]) {
+    const uint32_t gi = c - 1u;
+    float w[12];
+    for (int k = 0; k < 12; ++k) w[k] = bitsf((uint32_t)s.table[12 * (size_t)gi + k]);
+    const float sy = (float)(y & ~1) + ((y & 1) ? 1.5f : 0.5f);
+    float ax, ay, bx, by;
+    gradient_coord(ctx, w, (float)x0 + xoff, sy, ax, ay);
+    gradient_coord(ctx, w, (float)(x0 + wd) + xoff, sy, bx, by);
+    const float t = ((float)(x - x0) + 0.5f) / (float)wd;
+    const float omt = 1.f - t;
+    const float gx = ax * omt + bx * t, gy = ay * omt + by * t;
+    const float dx = gx - w[8], dy = gy - w[9];
+    const float A = dx * dx + dy * dy, B = dx * w[8] + dy * w[9];
+    const float den = std::sqrt(B * B + A * w[10]) - B;
+    float cc = A * (den > 1e-6f ? 1.f / den : 0.f);
+    cc = cc < 1.f ? cc : 1.f;
+    if (s.rows == 0) {
+        out[0] = out[1] = out[2] = 0.f;
+        out[3] = 1.f;
+        return;
+    }
+    const int32_t ri = s.table[12 * (size_t)gi + 6];
+    const float irh = 1.f / (float)s.rows;
+    const float u = ((float)(ri & 1023) + 0.5f) * (1.f / 1024.f) + cc * w[7];
+    const float v = ((float)(ri >> 10) + 0.5f) * irh;
+    const float tu = u * 1024.f - 0.5f, tv = v * (float)s.rows - 0.5f;
+    const float fu = std::floor(tu), fv = std::floor(tv);
+    const float au = tu - fu, av = tv - fv, bu = 1.f - au, bv = 1.f - av;
+    const int iu = f2i_rz(fu), jv = f2i_rz(fv);
+    const int rmax = (int)s.rows - 1;
+    const int i0 = std::min(std::max(iu, 0), 1023), i1 = std::min(std::max(iu + 1, 0), 1023);
+    const int j0 = std::min(std::max(jv, 0), rmax), j1 = std::min(std::max(jv + 1, 0), rmax);
+    auto tex = [&](int i, int j, int ch) {
+        const uint32_t v8 = ((uint32_t)s.ramp[(size_t)j * 1024 + i] >> (8 * ch)) & 0xFFu;
+        return ch < 3 ? dc[v8] : da[v8];
+    };
+    for (int ch = 0; ch < 4; ++ch)
+        out[ch] = (tex(i0, j0, ch) * bu + tex(i1, j0, ch) * au) * bv + (tex(i0, j1, ch) * bu + tex(i1, j1, ch) * au) * av;
+}
+
+// ---------------------------------------------------------------------------------
 // the whole pipeline
 // ---------------------------------------------------------------------------------
 void run_pipeline(const Scene &s, const Tables &T, const float *m, int W, int H, int srgb, int ms,
@@ -1034,6 +1157,8 @@ void run_pipeline(const Scene &s, const Tables &T, const float *m, int W, int H,
     float dec_c[256], thr_c[256], dec_a[256], thr_a[256];
     color_tables(srgb, dec_c, thr_c);
     color_tables(0, dec_a, thr_a);
+    float gctx[13] = {0};
+    if (s.gslots) inverse_projection(m, W, H, gctx);
     const int nbands = (H + 1) / 2;
     std::vector<int64_t> bcount(nbands + 1, 0);
     for (int64_t k = 0; k < M + S; ++k) {
@@ -1064,8 +1189,10 @@ void run_pipeline(const Scene &s, const Tables &T, const float *m, int W, int H,
             int wdt = d[1];
             uint32_t col = (uint32_t)d[2];
             const uint32_t *mk = d[3] ? &r.merged_mask[4 * (size_t)(d[3] - 1)] : nullptr;
-            const float sa = dec_a[(col >> 24) & 0xFF];
-            const float sc[3] = {dec_c[col & 0xFF], dec_c[(col >> 8) & 0xFF], dec_c[(col >> 16) & 0xFF]};
+            const bool grad = !(col & 0xFF000000u) && gradient_paints(s, col);
+            if (!(col & 0xFF000000u) && !grad) continue;  // paints nothing
+            float sa = dec_a[(col >> 24) & 0xFF];
+            float sc[3] = {dec_c[col & 0xFF], dec_c[(col >> 8) & 0xFF], dec_c[(col >> 16) & 0xFF]};
             for (int dy = 0; dy < rows; ++dy) {
                 for (int dx = 0; dx < wdt; ++dx) {
                     int px = x + dx;
@@ -1081,6 +1208,12 @@ void run_pipeline(const Scene &s, const Tables &T, const float *m, int W, int H,
                         bits = mk[dx * 2 + dy];
                     }
                     uint8_t *px4 = &st[(((size_t)dy * W + px) * spp_state) * 4];
+                    if (grad) {
+                        float g4[4];
+                        gradient_paint(s, gctx, ms ? 0.f : 1.f, col, x, wdt, px, y0 + dy, dec_c, dec_a, g4);
+                        sc[0] = g4[0]; sc[1] = g4[1]; sc[2] = g4[2];
+                        sa = g4[3];
+                    }
                     if (!ms) {
                         int cnt = __builtin_popcount(bits);
                         float a = sa * ((float)cnt / (float)N);
@@ -1172,8 +1305,10 @@ oracle_result *oracle_run(uint32_t n_paths, uint32_t n_curves, uint32_t n_vertic
                           const uint8_t *curve_type, const uint32_t *curve_path_id,
                           const float *curve_arc_w, const uint8_t *fill_rule,
                           const uint32_t *fill_color, int width, int height, int samples,
-                          int srgb, int ms_output, const float *m16, int flags) {
+                          int srgb, int ms_output, const float *m16, int flags, const int32_t *ramp,
+                          uint32_t n_ramp, const int32_t *table, uint32_t n_table) {
     if (samples != 8 && samples != 16 && samples != 32) return nullptr;
+    if ((n_ramp && !ramp) || (n_table && !table)) return nullptr;
     if (width <= 0 || height <= 0 || width > 32766 || height > 32766) return nullptr;
     Scene s;
     s.P = n_paths; s.C = n_curves; s.V = n_vertices;
@@ -1186,10 +1321,22 @@ oracle_result *oracle_run(uint32_t n_paths, uint32_t n_curves, uint32_t n_vertic
     for (uint32_t i = 0; i < s.C; ++i)
         if (s.ctype[i] == CT_RATIONAL && curve_arc_w) s.cw[i] = curve_arc_w[i];
     s.frule.assign(fill_rule, fill_rule + s.P);
+    // fill words (ras_base.cpp:144-166): colours keep their value unless alpha is 0; with a
+    // gradient table, a word in [1, 0x01000000] names gradient slot word - 1
+    const uint32_t gslots = n_table / 12;
+    bool any_grad = false;
     s.finfo.resize(s.P);
-    for (uint32_t p = 0; p < s.P; ++p) {  // fill_info rule (ras_base.cpp:147-151)
+    for (uint32_t p = 0; p < s.P; ++p) {
         uint32_t c = fill_color[p];
-        s.finfo[p] = (c & 0xFF000000u) ? c : 0u;
+        const bool g = !(c & 0xFF000000u) && gslots && c != 0u;
+        s.finfo[p] = ((c & 0xFF000000u) || g) ? c : 0u;
+        any_grad |= g;
+    }
+    if (any_grad) {
+        s.gslots = gslots;
+        s.rows = n_ramp >> 10;
+        s.table.assign(table, table + (size_t)gslots * 12);
+        s.ramp.assign(ramp, ramp + (size_t)s.rows * 1024);
     }
     s.vpath.assign(s.V, 0);
     for (uint32_t c = 0; c < s.C; ++c)

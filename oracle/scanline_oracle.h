@@ -21,15 +21,18 @@ extern "C" {
 
 typedef struct oracle_result oracle_result;
 
-/* Run the whole pipeline.  Arrays follow include/scanline.h's sl_scene.  `m16` is the
- * column-major input transform (NULL = identity).  flags bit 0: keep per-stage
- * intermediates (needed by oracle_get for everything but "image"). */
+/* Run the whole pipeline.  Arrays follow include/scanline.h's sl_scene (gradient ramp and
+ * table textures last; NULL with size 0 when absent).  `m16` is the column-major input
+ * transform (NULL = identity).  flags bit 0: keep per-stage intermediates (needed by
+ * oracle_get for everything but "image"). */
 oracle_result *oracle_run(uint32_t n_paths, uint32_t n_curves, uint32_t n_vertices,
                           const float *vertex, const uint32_t *curve_vertex_pos,
                           const uint8_t *curve_type, const uint32_t *curve_path_id,
                           const float *curve_arc_w, const uint8_t *fill_rule,
                           const uint32_t *fill_color, int width, int height, int samples,
-                          int srgb, int ms_output, const float *m16, int flags);
+                          int srgb, int ms_output, const float *m16, int flags,
+                          const int32_t *ramp, uint32_t n_ramp, const int32_t *table,
+                          uint32_t n_table);
 
 /* Borrow one named array of a result (see oracle_names for the list). */
 int oracle_get(const oracle_result *r, const char *name, const void **ptr, size_t *nbytes);

@@ -60,11 +60,17 @@ def test_a128_and_gradients():
     vg, _, _ = rvg.loads(text, a128=True)
     col = vg.arrays().fill_color.astype(np.uint32)
     assert (col[1] >> 24) == int(np.trunc(f32(int(np.trunc(f32(127) / f32(255) * f32(127)))) / f32(127) * f32(255)))
-    grad = text.replace("solid rgba(0,0.5,1,1)", "linear 0,0 1,1 pad")
+    grad = text.replace("solid rgba(0,0.5,1,1)", "linear 0,0 1,1 0,rgba(0,0,0,1) 1,rgba(1,1,1,1) pad 1,0,0,0,1,0")
     with pytest.raises(rvg.RVGError):
-        rvg.loads(grad)
+        rvg.loads(grad, gradients="raise")
     vg, _, _ = rvg.loads(grad, gradients="skip")
     assert vg.arrays().fill_color[1] == 0
+    # rendered: the fill word names the path's table slot (path index + 1)
+    vg, _, _ = rvg.loads(grad)
+    assert vg.arrays().fill_color[1] == 2
+    # a gradient line without its transform numbers is refused
+    with pytest.raises(rvg.RVGError):
+        rvg.loads(text.replace("solid rgba(0,0.5,1,1)", "linear 0,0 1,1 pad"))
 
 
 def test_unsupported_command_raises():

@@ -124,9 +124,12 @@ def test_gradient_fill_and_stroke_override():
     grad = ('<defs><linearGradient id="g" x1="0" y1="0" x2="1" y2="0" gradientUnits="userSpaceOnUse">'
             '<stop offset="0" stop-color="#000"/></linearGradient></defs>')
     with pytest.raises(svg.SVGError):
-        svg.loads(doc(grad + '<path d="M 0 0 L 1 0" fill="url(#g)"/>'))
+        svg.loads(doc(grad + '<path d="M 0 0 L 1 0" fill="url(#g)"/>'), gradients="raise")
+    # the fill word names the path's gradient slot (path index + 1); skipped, no textures
     vg = svg.loads(doc(grad + '<path d="M 0 0 L 1 0" fill="url(#g)"/>'), gradients="skip")
-    assert vg._ftype == [FT_GRADIENT] and vg.fillInfo() == [0]
+    assert vg._ftype == [FT_GRADIENT] and vg.fillInfo() == [1] and vg.gradient_table == []
+    vg = svg.loads(doc(grad + '<path d="M 0 0 L 1 0" fill="url(#g)"/>'))
+    assert len(vg.gradient_table) == 12 and len(vg.gradient_ramp) == 1024
     # a colour stroke becomes the fill word, so this path renders without the gradient
     vg = svg.loads(doc(grad + '<path d="M 0 0 L 1 0" fill="url(#g)" stroke="#0f0"/>'))
     assert vg._stype == [ST_COLOR] and vg.fillInfo() == [rgba(0, 255, 0)]

@@ -200,10 +200,11 @@ def test_gradient_paths(tmp_path):
     fn = str(tmp_path / "g.vgb")
     vg.save(fn)
     with pytest.raises(vgfile.VGFileError):
-        VGContainer.load(fn)
-    g = VGContainer.load(fn, gradients="skip")
+        VGContainer.load(fn, gradients="raise")
+    g = VGContainer.load(fn)
     assert g.gradient_ramp == [1, 2, 3] and g.gradient_table == [-4, 5]
-    assert [int(c) for c in g.arrays().fill_color] == [rgba(255, 0, 0, 255), 0]  # draws nothing
+    # the gradient path's fill word names its table slot (path index + 1)
+    assert [int(c) for c in g.arrays().fill_color] == [rgba(255, 0, 0, 255), 2]
     fn2 = str(tmp_path / "g2.vgt")
     g.save(fn2)
     assert VGContainer.load(fn2, gradients="skip")._ftype == [FT_COLOR, FT_GRADIENT]
