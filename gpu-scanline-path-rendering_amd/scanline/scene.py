@@ -89,6 +89,7 @@ class VGContainer:
         self._v, self._cpos, self._ctype, self._cpath, self._cw = [], [], [], [], []
         self._rule, self._color, self._ftype, self._findex = [], [], [], []
         self._stype, self._scolor = [], []  # stroke type and colour per path
+        self._swidth, self._sindex = [], []  # stroke width and gradient index per path
         self._cfirst, self._cowner = [], []  # contours: first curve, owning path
         self.gradient_ramp, self.gradient_table = [], []  # kept for file round trips
 
@@ -113,7 +114,29 @@ class VGContainer:
         self._findex.append(int(f_index) & 0xFFFFFFFF)
         self._stype.append(ST_NONE)
         self._scolor.append(rgba(0, 0, 0))
+        self._swidth.append(1.0)
+        self._sindex.append(0xFFFFFFFF)
         return len(self._rule) - 1
+
+    def addStrokePath(self, s_color, width):
+        """addStrokePath (vg_container.cpp:322-332): a path with no fill and a colour stroke
+        of the given width."""
+        pid = self.addPath(0, rgba(0, 0, 0), FT_NONE)
+        self._stype[pid] = ST_COLOR
+        self._scolor[pid] = int(s_color)
+        self._swidth[pid] = float(width)
+        return pid
+
+    def contours(self):
+        """Curve index ranges [first, last) of each contour, with its path."""
+        ends = self._cfirst[1:] + [len(self._ctype)]
+        return [(f, e, o) for f, e, o in zip(self._cfirst, ends, self._cowner)]
+
+    def strokeToFill(self):
+        """strokeToOffsetCurve (vg_container.cpp:555-1353): a container in which every stroked
+        path is followed by a nonzero fill path of its stroke outline; see scanline.stroke."""
+        from . import stroke
+        return stroke.stroke_to_fill(self)
 
     def addContour(self):
         """addContour (vg_container.cpp:269-280): the next curves form a new contour of the
@@ -198,6 +221,8 @@ class VGContainer:
         vg._findex = [0xFFFFFFFF] * len(vg._rule)
         vg._stype = [ST_NONE] * len(vg._rule)
         vg._scolor = [rgba(0, 0, 0)] * len(vg._rule)
+        vg._swidth = [1.0] * len(vg._rule)
+        vg._sindex = [0xFFFFFFFF] * len(vg._rule)
         owners, first = np.unique(pid, return_index=True)
         vg._cfirst, vg._cowner = first.tolist(), owners.tolist()
         return vg

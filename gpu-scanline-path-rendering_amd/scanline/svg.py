@@ -37,7 +37,10 @@ values (`rgba(...)` and percentage colours, `circle` / `ellipse` without x / y /
 height, a `color` attribute, which pops the reference's attribute stack, unparseable
 numbers) raise `SVGError` with no partial result.
 
-Colour strokes fill the path with the stroke colour (see `VGContainer.fillInfo`).
+Colour strokes fill the path with the stroke colour (see `VGContainer.fillInfo`), as the
+reference's renderer draws its input; with `stroke_to_fill=True` every stroked path is
+instead turned into an outline path of its stroke width (`scanline.stroke`), drawn after its
+fill.
 
 Gradient fills.  The reference's SVG loader keeps the gradient definitions but builds no
 gradient textures (only its RVG loader does, rvg.cpp:490-758), so its renderer reads an
@@ -439,7 +442,8 @@ def _path_data(vg, d):
 # ---------------------------------------------------------------------------------
 def _new_attr():
     return dict(fill=FT_COLOR, rule=0, color=(0, 0, 0, 255), findex=0xFFFFFFFF, fopacity=F(1),
-                stroke=ST_NONE, scolor=(0, 0, 0, 255), sindex=0xFFFFFFFF, mat=_identity())
+                stroke=ST_NONE, scolor=(0, 0, 0, 255), sindex=0xFFFFFFFF, swidth=F(1), sopacity=F(1),
+                mat=_identity())
 
 
 def _copy_attr(a):
@@ -489,7 +493,11 @@ class _Loader:
             else:
                 top["color"] = _color(v)
                 top["fill"] = FT_COLOR
-        elif name in ("opacity", "stroke-opacity", "stroke-width", "stroke-miterlimit"):
+        elif name == "stroke-width":
+            top["swidth"] = _to_float(v, name)
+        elif name == "stroke-opacity":
+            top["sopacity"] = _to_float(v, name)
+        elif name in ("opacity", "stroke-miterlimit"):
             _to_float(v, name)
         elif name == "stroke-linecap":
             if v not in ("butt", "round", "square"):
@@ -563,8 +571,10 @@ class _Loader:
         vg._findex[pi] = a["findex"]
         vg._stype[pi] = a["stroke"]
         vg._scolor[pi] = rgba(*a["scolor"])
+        vg._swidth[pi] = float(a["swidth"])
+        vg._sindex[pi] = a["sindex"]
         m = a["mat"]
-        self.pmat[pi] = (m, a["fopacity"])
+        self.pmat[pi] = (m, a["fopacity"], a["sopacity"])
         for k in range(vbegin, vg.vertexNumber()):
             x, y = _apply(m, *vg._v[k])
             vg._v[k] = (float(x), float(y))
@@ -723,19 +733,22 @@ def _affine(m):
     return np.array([[m[0][0], m[1][0], m[2][0]], [m[0][1], m[1][1], m[2][1]], [0.0, 0.0, 1.0]], dtype=np.float64)
 
 
-def _gradient_textures(L, srgb):
-    """Gradient table and ramp for the loaded paths (see the module notes)."""
-    vg = L.vg
+def _gradient_textures(L, vg, srgb):
+    """Gradient table and ramp for the paths of `vg`, the loaded container or its
+    stroke-to-fill conversion (see the module notes)."""
     g = GradientTextures(srgb)
     vx, vy = float(L.viewbox[0]), float(L.viewbox[1])
     hf = float(F(vg.height & 0xFFFFFFFF))
     final = np.array([[1.0, 0.0, -vx], [0.0, -1.0, hf + vy], [0.0, 0.0, 1.0]])
+    source = getattr(vg, "source", None) or [(pi, "fill") for pi in range(vg.pathNumber())]
     for pi in range(vg.pathNumber()):
-        if vg._ftype[pi] != FT_GRADIENT or vg._stype[pi] == ST_COLOR or pi not in L.pmat:
+        orig, kind = source[pi]
+        if vg._ftype[pi] != FT_GRADIENT or vg._stype[pi] == ST_COLOR or orig not in L.pmat:
             g.add_solid()
             continue
         gd = vg.svg_gradients[vg._findex[pi]]
-        m, opacity = L.pmat[pi]
+        m, fop, sop = L.pmat[orig]
+        opacity = sop if kind == "stroke" else fop
         tr = final @ _affine(m) @ _affine(gd["transform"])
         stops = [(float(st["offset"]),) + tuple(c / 255.0 for c in st["color"]) for st in gd["stops"]]
         words = [tr[0, 0], tr[0, 1], tr[0, 2], tr[1, 0], tr[1, 1], tr[1, 2]]
@@ -750,8 +763,10 @@ def _gradient_textures(L, srgb):
     vg.gradient_table = table.astype(np.int64).tolist()
 
 
-def loads(text, a128=False, gradients="render", on_error="raise", srgb=True):
-    """Parse SVG text into a VGContainer (see the module notes)."""
+def loads(text, a128=False, gradients="render", on_error="raise", srgb=True, stroke_to_fill=False):
+    """Parse SVG text into a VGContainer (see the module notes).  stroke_to_fill: turn every
+    stroked path into fill geometry (VGContainer.strokeToFill) instead of filling it with
+    the stroke colour."""
     if gradients not in ("render", "raise", "skip") or on_error not in ("raise", "partial"):
         raise ValueError("gradients: 'render', 'raise' or 'skip'; on_error: 'raise' or 'partial'")
     L = _Loader(a128, gradients)
@@ -760,11 +775,13 @@ def loads(text, a128=False, gradients="render", on_error="raise", srgb=True):
     if stopped is not None and on_error == "raise":
         raise SVGError("the reference's SVG::load stops here: %s" % stopped, partial=vg)
     # a colour stroke replaces the fill word, so only gradient fills without one need a gradient
+    if stroke_to_fill:
+        vg = vg.strokeToFill()
     used = any(t == FT_GRADIENT and st != ST_COLOR for t, st in zip(vg._ftype, vg._stype))
     if gradients == "raise" and used:
         raise SVGError("gradient paints refused (gradients='raise')")
     if gradients == "render" and used:
-        _gradient_textures(L, srgb)
+        _gradient_textures(L, vg, srgb)
     return vg
 
 

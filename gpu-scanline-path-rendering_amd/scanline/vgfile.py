@@ -127,6 +127,8 @@ def _container(f, gradients):
     vg._findex = f["p_findex"].astype(np.int64).tolist()
     vg._stype = [ST_NONE] * n_path  # the files hold no stroke fields
     vg._scolor = [rgba(0, 0, 0)] * n_path
+    vg._swidth = [1.0] * n_path
+    vg._sindex = [0xFFFFFFFF] * n_path
     vg._cpos = vpos.tolist()
     vg._ctype = ctype.astype(int).tolist()
     vg._cpath = np.repeat(con_owner, ccn).tolist()
