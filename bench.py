@@ -67,8 +67,6 @@ STAGE_KERNEL = {("composite", False): "k_composite_cov", ("composite", True): "k
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plain_2048_32spp_traffic.md")
 INSTR_PROFILE = os.path.join(ROOT, "profiles", "plain_2048_32spp_instructions.md")
-# VALU issue ceiling: 1024 SIMDs (256 CUs x 4) at 2.4 GHz, 4 cycles per wave64 instruction
-VALU_WAVE_INSTS_PER_S = 1024 * 2.4e9 / 4
 
 
 def _profile_value(path, key, kernel):
@@ -397,13 +395,16 @@ def main():
     dom_bytes = stage_bytes(dom, cnt, args.spp, W, H, args.ms)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
     dom_kernel = STAGE_KERNEL.get((dom, args.ms), dom)
-    traffic = valu_frac = None
+    traffic = valu_frac = valu_ceiling = None
     if args.config == 2 and not args.ms and (W, args.paths, args.spp) == (2048, 20000, 32):
         traffic = profiled_traffic(dom_kernel)
         valu = profiled_valu(dom_kernel)
         if valu:
-            # issued VALU instructions over what the SIMDs can issue in the kernel's time
-            valu_frac = round(valu / (VALU_WAVE_INSTS_PER_S * stages[dom] * 1e-3), 3)
+            # issued VALU instructions over what the SIMDs issued in the fma probe (measured
+            # now, on this GPU and clock), scaled to the kernel's time
+            pm, pn = r.valuProbe()
+            valu_ceiling = pn / (pm * 1e-3)
+            valu_frac = round(valu / (valu_ceiling * stages[dom] * 1e-3), 3)
     b_alg = b_alg_total(cnt, args.spp, W, H)
 
     cpu = None
@@ -436,7 +437,8 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "alg_bytes": int(dom_bytes), "traffic": traffic, "valu_issue_frac": valu_frac},
+                     "alg_bytes": int(dom_bytes), "traffic": traffic, "valu_issue_frac": valu_frac,
+                     "valu_ceiling_wave_insts_per_s": valu_ceiling and float(f"{valu_ceiling:.4g}")},
         "cpu_baseline": cpu,
     }
     if cpu:

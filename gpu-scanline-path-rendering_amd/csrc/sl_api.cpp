@@ -57,6 +57,7 @@ struct sl_handle {
     int srgb = 1;
     float m[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};  // column-major
     int W = 0, H = 0;
+    int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;  // presentation rectangle (0 size: the image's)
     bool has_scene = false;
     uint32_t P = 0, C = 0, V = 0;
     std::string err;
@@ -746,6 +747,59 @@ int sl_resize_output_buffer(sl_handle *h, int width, int height) {
     return SL_OK;
 }
 
+// The image of the last frame, if the output size is still the one it was rendered at.
+static int check_image(sl_handle *h) {
+    if (!h->A.b_image.p || h->img_W <= 0) return fail(h, SL_E_STATE, "no image rendered yet");
+    if (h->img_W != h->W || h->img_H != h->H)
+        return fail(h, SL_E_STATE, "output size changed to %dx%d since the last frame (%dx%d): rasterize first",
+                    h->W, h->H, h->img_W, h->img_H);
+    return SL_OK;
+}
+
+int sl_viewport(sl_handle *h, int x, int y, int width, int height) {
+    if (!h) return SL_E_ARG;
+    if (width < 0 || height < 0) return fail(h, SL_E_ARG, "viewport size %dx%d is negative", width, height);
+    h->vp_x = x;
+    h->vp_y = y;
+    h->vp_w = width;
+    h->vp_h = height;
+    return SL_OK;
+}
+
+int sl_resize_viewport(sl_handle *h, int width, int height) {
+    if (!h) return SL_E_ARG;
+    if (width < 0 || height < 0) return fail(h, SL_E_ARG, "viewport size %dx%d is negative", width, height);
+    h->vp_w = width;
+    h->vp_h = height;
+    return SL_OK;
+}
+
+int sl_present(sl_handle *h, void *dev_dst, int dst_pitch, int dst_width, int dst_height, void *stream) {
+    if (!h || dst_width < 0 || dst_height < 0) return SL_E_ARG;
+    if (!dev_dst && dst_width && dst_height) return fail(h, SL_E_ARG, "destination pointer is NULL");
+    if (dst_pitch < dst_width * 4) return fail(h, SL_E_ARG, "pitch %d is below width * 4", dst_pitch);
+    int rc = check_image(h);
+    if (rc) return rc;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    // the source rectangle (0, 0, vp_w, vp_h) of the image goes to (vp_x, vp_y), clipped
+    // to both the image and the destination, one pixel for one (GL_NEAREST, same size)
+    const int vw = h->vp_w ? h->vp_w : h->img_W, vh = h->vp_h ? h->vp_h : h->img_H;
+    int sx0 = 0, sy0 = 0, dx0 = h->vp_x, dy0 = h->vp_y;
+    int w = std::min(vw, h->img_W), hh = std::min(vh, h->img_H);
+    if (dx0 < 0) { sx0 -= dx0; w += dx0; dx0 = 0; }
+    if (dy0 < 0) { sy0 -= dy0; hh += dy0; dy0 = 0; }
+    w = std::min(w, dst_width - dx0);
+    hh = std::min(hh, dst_height - dy0);
+    if (w <= 0 || hh <= 0) return SL_OK;
+    const hipStream_t s = (hipStream_t)stream;
+    if (h->pending && s != h->pending_stream) HIPCHK(h, hipStreamWaitEvent(s, h->done_ev, 0));
+    const size_t spitch = (size_t)h->img_W * 4;
+    HIPCHK(h, hipMemcpy2DAsync((uint8_t *)dev_dst + (size_t)dy0 * dst_pitch + (size_t)dx0 * 4, (size_t)dst_pitch,
+                               (const uint8_t *)h->A.b_image.p + (size_t)sy0 * spitch + (size_t)sx0 * 4, spitch,
+                               (size_t)w * 4, (size_t)hh, hipMemcpyDeviceToDevice, s));
+    return SL_OK;
+}
+
 int sl_add_vg(sl_handle *h, const sl_scene *s) {
     if (!h) return SL_E_ARG;
     h->sized = false;
@@ -1026,15 +1080,6 @@ int sl_rasterize_noalloc(sl_handle *h, void *dev_rgba, int pitch_bytes, void *st
     return SL_OK;
 }
 
-// The image of the last frame, if the output size is still the one it was rendered at.
-static int check_image(sl_handle *h) {
-    if (!h->A.b_image.p || h->img_W <= 0) return fail(h, SL_E_STATE, "no image rendered yet");
-    if (h->img_W != h->W || h->img_H != h->H)
-        return fail(h, SL_E_STATE, "output size changed to %dx%d since the last frame (%dx%d): rasterize first",
-                    h->W, h->H, h->img_W, h->img_H);
-    return SL_OK;
-}
-
 int sl_output_device_ptr(sl_handle *h, void **dev_ptr, int *pitch_bytes) {
     if (!h || !dev_ptr || !pitch_bytes) return SL_E_ARG;
     if (int rc = check_image(h)) return rc;
@@ -1114,6 +1159,32 @@ int sl_debug_requantize(sl_handle *h, int channel, const float *values, int64_t 
     (void)hipFree(dc);
     (void)hipFree(dd);
     if (e != hipSuccess) return fail(h, SL_E_HIP, "requantize: %s", hipGetErrorString(e));
+    return SL_OK;
+}
+
+int sl_debug_valu_probe(sl_handle *h, int blocks, int iters, double *ms, uint64_t *wave_insts) {
+    if (!h || blocks <= 0 || iters <= 0 || !ms || !wave_insts) return SL_E_ARG;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    void *out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&out, (size_t)blocks * 4);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        sl::launch_valu_probe(blocks, iters, (float *)out, nullptr);  // warm-up
+        (void)hipEventRecord(e0, nullptr);
+        sl::launch_valu_probe(blocks, iters, (float *)out, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        e = hipEventSynchronize(e1);
+    }
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    if (e != hipSuccess) return fail(h, SL_E_HIP, "valu probe: %s", hipGetErrorString(e));
+    *ms = t;
+    *wave_insts = (uint64_t)blocks * 4ull * (uint64_t)iters * 512ull;
     return SL_OK;
 }
 

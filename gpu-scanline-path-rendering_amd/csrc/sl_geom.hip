@@ -107,6 +107,27 @@ __global__ void k_fill_rows_u8(uint8_t *p, uint8_t v, size_t row_bytes, size_t p
 
 inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// VALU issue-rate probe: every thread runs `iters` rounds of 64 steps of 8 independent fma
+// chains, i.e. 512 v_fma_f32 per round and wave, nothing else in the loop body but the
+// loop counter (scalar).  Timed over a full grid it gives the wave VALU instructions per
+// second the machine issues.
+__global__ void __launch_bounds__(256) k_valu_probe(int iters, float seed, float *out) {
+    float a0 = seed + threadIdx.x, a1 = a0 + 1.f, a2 = a0 + 2.f, a3 = a0 + 3.f;
+    float a4 = a0 + 4.f, a5 = a0 + 5.f, a6 = a0 + 6.f, a7 = a0 + 7.f;
+    const float m = 0.999f, c = 1e-3f;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {
+            a0 = __builtin_fmaf(a0, m, c); a1 = __builtin_fmaf(a1, m, c);
+            a2 = __builtin_fmaf(a2, m, c); a3 = __builtin_fmaf(a3, m, c);
+            a4 = __builtin_fmaf(a4, m, c); a5 = __builtin_fmaf(a5, m, c);
+            a6 = __builtin_fmaf(a6, m, c); a7 = __builtin_fmaf(a7, m, c);
+        }
+    }
+    const float r = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
+    if (r == 12345.f) out[blockIdx.x] = r;  // keeps the chains live; never true in practice
+}
+
 // ------------------------------------------------------------------------------------
 // per-path init, vertex transform + region flags (k_transformVertex, .cu:36-105)
 // ------------------------------------------------------------------------------------
@@ -889,6 +910,10 @@ void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
 }
+void launch_valu_probe(int blocks, int iters, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, s, iters, 0.5f, out);
+}
+
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);

@@ -175,6 +175,7 @@ void launch_encode_values(const Tables &t, int channel, const float *v, int64_t 
 void launch_encode_sweep(const Tables &t, int channel, uint32_t lo, uint32_t hi, unsigned long long *bad,
                          hipStream_t s);
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+void launch_valu_probe(int blocks, int iters, float *out, hipStream_t s);
 void launch_fill_rows_u8(uint8_t *p, uint8_t v, size_t row_bytes, size_t pitch, size_t rows, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 

@@ -18,10 +18,10 @@ FILL_NONZERO, FILL_EVEN_ODD = 0, 1
 EXPORTS = (
     "sl_create", "sl_destroy", "sl_set_samples", "sl_set_multisample_output",
     "sl_enable_srgb_correction", "sl_set_input_transform", "sl_resize_output_buffer",
-    "sl_add_vg", "sl_clear", "sl_rasterize", "sl_rasterize_noalloc", "sl_output_device_ptr", "sl_read_output",
+    "sl_viewport", "sl_resize_viewport", "sl_present", "sl_add_vg", "sl_clear", "sl_rasterize", "sl_rasterize_noalloc", "sl_output_device_ptr", "sl_read_output",
     "sl_element_number", "sl_enable_step_timing", "sl_step_timing", "sl_step_name",
     "sl_pixel_count", "sl_last_error", "sl_debug_enable", "sl_debug_dump",
-    "sl_debug_requantize", "sl_debug_requantize_sweep", "sl_batch_create", "sl_batch_destroy",
+    "sl_debug_requantize", "sl_debug_requantize_sweep", "sl_debug_valu_probe", "sl_batch_create", "sl_batch_destroy",
     "sl_batch_render", "sl_batch_render_eager", "sl_batch_tune", "sl_batch_set_shape", "sl_batch_get_shape",
     "sl_batch_element_number", "sl_batch_output_device_ptr", "sl_batch_read_output", "sl_batch_last_error",
     "sl_diff_create", "sl_diff_destroy", "sl_diff_forward", "sl_diff_backward", "sl_diff_crossings",
@@ -86,6 +86,9 @@ def lib():
         "sl_enable_srgb_correction": (i, [vp, i]),
         "sl_set_input_transform": (i, [vp, ctypes.POINTER(ctypes.c_float)]),
         "sl_resize_output_buffer": (i, [vp, i, i]),
+        "sl_viewport": (i, [vp, i, i, i, i]),
+        "sl_resize_viewport": (i, [vp, i, i]),
+        "sl_present": (i, [vp, vp, i, i, i, vp]),
         "sl_add_vg": (i, [vp, ctypes.POINTER(SlScene)]),
         "sl_clear": (i, [vp]),
         "sl_rasterize": (i, [vp, vp]),
@@ -102,6 +105,7 @@ def lib():
         "sl_debug_dump": (i, [vp, ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
         "sl_debug_requantize": (i, [vp, i, vp, ctypes.c_int64, vp, vp]),
         "sl_debug_requantize_sweep": (i, [vp, i, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(u64)]),
+        "sl_debug_valu_probe": (i, [vp, i, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)]),
         "sl_batch_create": (i, [i, vp, vp, vp, vp, i, i, ctypes.POINTER(vp)]),
         "sl_batch_destroy": (i, [vp]),
         "sl_batch_render": (i, [vp, vp]),

@@ -76,6 +76,19 @@ class VGRasterizer:
     def outputBufferSize(self):
         return self._w, self._h_px
 
+    def viewport(self, x, y, width, height):
+        """viewport (ras_base.cpp:385-388): where the image lands in the window."""
+        self._chk(self._L.sl_viewport(self._h, int(x), int(y), int(width), int(height)))
+
+    def resizeViewport(self, width, height):
+        """resizeViewport (ras_base.cpp:391-394)."""
+        self._chk(self._L.sl_resize_viewport(self._h, int(width), int(height)))
+
+    def present(self, dev_ptr, pitch, width, height, stream=None):
+        """The window blit (ras_base.cpp:1170-1176) into a caller's device RGBA8 buffer."""
+        self._chk(self._L.sl_present(self._h, ctypes.c_void_p(dev_ptr), int(pitch), int(width), int(height),
+                                     ctypes.c_void_p(stream or 0)))
+
     def enableStepTiming(self, f=True):
         self._chk(self._L.sl_enable_step_timing(self._h, int(bool(f))))
 
@@ -155,6 +168,12 @@ class VGRasterizer:
         self._chk(self._L.sl_debug_requantize_sweep(self._h, int(channel), int(bits_lo), int(bits_hi),
                                                     ctypes.byref(out)))
         return out.value
+
+    def valuProbe(self, blocks=8192, iters=64):
+        """(ms, wave VALU instructions) of one timed launch of the fma issue-rate probe."""
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        self._chk(self._L.sl_debug_valu_probe(self._h, int(blocks), int(iters), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def dump(self, name):
         """Parity dump of one intermediate array in the reference's layout."""

@@ -128,6 +128,19 @@ int sl_set_input_transform(sl_handle *h, const float m[16]);
 /* resizeOutputBuffer (ras_base.cpp:393-402). */
 int sl_resize_output_buffer(sl_handle *h, int width, int height);
 
+/* viewport / resizeViewport (ras_base.h:147-148, ras_base.cpp:385-394): the rectangle of
+ * the final blit of the output image into the window (ras_base.cpp:1170-1176: source
+ * (0, 0, w, h) to destination (x, y, x + w, y + h), GL_NEAREST, no scaling).  A size of
+ * 0 (the default) means the image's size. */
+int sl_viewport(sl_handle *h, int x, int y, int width, int height);
+int sl_resize_viewport(sl_handle *h, int width, int height);
+
+/* The window blit itself, into a caller's device RGBA8 buffer of dst_width x dst_height
+ * pixels with rows dst_pitch bytes apart: the viewport rectangle of the last image copied
+ * one pixel for one to (x, y), clipped to the image and to the buffer; rows in the same
+ * y-up order as the image.  Queued on `stream` after the last frame. */
+int sl_present(sl_handle *h, void *dev_dst, int dst_pitch, int dst_width, int dst_height, void *stream);
+
 /* ---- scene ------------------------------------------------------------------------ */
 
 /* addVg (ras_cut_mask_comb_scanline.cpp:224-229 -> ras_base.cpp:70-382) followed by
@@ -226,6 +239,12 @@ int sl_debug_requantize(sl_handle *h, int channel, const float *values, int64_t 
                         uint8_t *codes, float *decoded);
 int sl_debug_requantize_sweep(sl_handle *h, int channel, uint32_t bits_lo, uint32_t bits_hi,
                               uint64_t *mismatches);
+
+/* VALU issue-rate probe (no reference counterpart; calibrates the instruction ceiling the
+ * benchmark reports against): one timed launch of `blocks` workgroups of 256 threads, each
+ * running `iters` rounds of 512 independent v_fma_f32 per wave.  Returns the launch time
+ * and the wave VALU instructions it issued (blocks * 4 * iters * 512). */
+int sl_debug_valu_probe(sl_handle *h, int blocks, int iters, double *ms, uint64_t *wave_insts);
 
 /* ---- batched scenes -------------------------------------------------------------------
  * Many independent scenes of uneven size rendered by one launch sequence over all of

@@ -528,3 +528,37 @@ def test_output_after_resize_needs_a_new_frame():
             assert r.outputDevicePtr()[1] == w * 4
     finally:
         r.close()
+
+
+def test_viewport_present():
+    """viewport / resizeViewport place the image in a window buffer one pixel for one
+    (the reference's final blit, ras_base.cpp:1170-1176), clipped to both rectangles."""
+    import torch
+    r = scanline.VGRasterizer(0)
+    try:
+        scene = scanline.generate(scanline.scene.KIND_MIXED, 80, 60, 30, 0x5EED0650)
+        render_gpu(r, scene, 80, 60, 8)
+        img = r.output()
+        for (vx, vy, vw, vh), (WW, WH) in [((10, 5, 0, 0), (128, 96)), ((-7, -3, 50, 40), (64, 48)),
+                                          ((100, 90, 80, 60), (120, 100))]:
+            if vw:
+                r.viewport(vx, vy, vw, vh)
+            else:
+                r.viewport(vx, vy, 0, 0)
+            pitch = WW * 4 + 16
+            buf = torch.full((WH * pitch,), 7, dtype=torch.uint8, device="cuda")
+            r.present(buf.data_ptr(), pitch, WW, WH, torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            got = buf.cpu().numpy().reshape(WH, pitch)[:, :WW * 4].reshape(WH, WW, 4)
+            want = np.full((WH, WW, 4), 7, np.uint8)
+            w, h = (vw or 80), (vh or 60)
+            for y in range(h):
+                for x in range(w):
+                    dx, dy = vx + x, vy + y
+                    if 0 <= dx < WW and 0 <= dy < WH and x < 80 and y < 60:
+                        want[dy, dx] = img[y, x]
+            np.testing.assert_array_equal(got, want, err_msg=str((vx, vy, vw, vh)))
+        r.resizeViewport(20, 10)
+        r.viewport(0, 0, 20, 10)
+    finally:
+        r.close()
