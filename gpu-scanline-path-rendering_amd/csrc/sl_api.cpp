@@ -38,7 +38,7 @@ struct Arena {
     DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
         b_boff, b_bwsum, b_bcarry, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
         b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
-        b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_image, b_scan_tmp, b_lcount;
+        b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_rpath, b_ptab, b_image, b_scan_tmp, b_lcount;
     // carving state: non-null base = every array comes from [base, base + cap)
     uint8_t *carve_base = nullptr;
     size_t carve_cap = 0, carve_off = 0;
@@ -494,6 +494,7 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
         (rc = ensure_in(h, A, A.b_rs_hoff, (256 * rs_tiles + 1) * 4 + 16)) || (rc = ensure_in(h, A, A.b_rowbeg, R * 4)) ||
         (rc = ensure_in(h, A, A.b_rowend, R * 4)) || (rc = ensure_in(h, A, A.b_rrec, (size_t)NB * 16)) ||
         (rc = ensure_in(h, A, A.b_btile, (size_t)NB * 8)) || (rc = ensure_in(h, A, A.b_rtile, (size_t)NB * 8)) ||
+        (rc = ensure_in(h, A, A.b_rpath, f.ms ? (size_t)NB * 4 : 0)) || (rc = ensure_in(h, A, A.b_ptab, f.ms ? (size_t)P * 1024 : 0)) ||
         (rc = ensure_in(h, A, A.b_scan_tmp, scan_need * 4)))
         return rc;
     f.binfo = P_<int2>(A.b_binfo); f.bcount = P_<uint32_t>(A.b_bcount); f.boff = P_<uint32_t>(A.b_boff);
@@ -513,6 +514,8 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
     f.rrec = P_<uint4>(A.b_rrec);
     f.btile = P_<uint64_t>(A.b_btile);
     f.rtile = P_<uint64_t>(A.b_rtile);
+    f.rpath = f.ms ? P_<uint32_t>(A.b_rpath) : nullptr;
+    f.ptab = f.ms ? P_<uint32_t>(A.b_ptab) : nullptr;
     f.scan_tmp = P_<uint32_t>(A.b_scan_tmp);
 
     sl::launch_bucket_info(sc, f, s);
@@ -692,7 +695,7 @@ int sl_destroy(sl_handle *h) {
                       &h->A.b_rec_mask, &h->A.b_scratch, &h->A.b_items, &h->A.b_imask, &h->A.b_icov, &h->A.b_bicnt, &h->A.b_bmcnt, &h->A.b_bext,
                       &h->A.b_big_list, &h->A.b_small, &h->A.b_nz_flag, &h->A.b_nz_off, &h->A.b_nzk0, &h->A.b_nzk1, &h->A.b_nzv0,
                       &h->A.b_nzv1, &h->A.b_rs_hist, &h->A.b_rs_hoff, &h->A.b_rowbeg, &h->A.b_rowend, &h->A.b_image, &h->A.b_scan_tmp,
-                      &h->A.b_rrec, &h->A.b_btile, &h->A.b_rtile, &h->A.b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry,
+                      &h->A.b_rrec, &h->A.b_btile, &h->A.b_rtile, &h->A.b_rpath, &h->A.b_ptab, &h->A.b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry,
                       &h->gramp, &h->gtable};
     for (DevBuf *b : bufs) release(*b);
     if (h->stage) (void)hipHostFree(h->stage);

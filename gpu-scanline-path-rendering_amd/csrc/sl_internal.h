@@ -142,6 +142,8 @@ struct Frame {
     uint32_t *rowend;        // [R]
     uint4 *rrec;             // [NB] row-list records {extent, colour, items, item base}
     uint64_t *rtile;         // [NB] row-list tile masks
+    uint32_t *rpath;         // [NB] row-list path ids (multisample output)
+    uint32_t *ptab;          // [P][256] per-path blend tables (multisample output)
     uint8_t *image;          // [H * pitch]
     int pitch;
     int srgb, ms;

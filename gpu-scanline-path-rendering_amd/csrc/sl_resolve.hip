@@ -977,7 +977,7 @@ __device__ __forceinline__ float4 ramp_texel(const GradDev &G, int i, int j, con
 
 // Paint of gradient word c at pixel (x, yrow) of an item covering columns [x0, x0 + w) of
 // the cell row holding yrow.  dc / da decode a texel's colour and alpha codes.
-__device__ float4 grad_color(const GradDev &G, uint32_t c, int x0, int w, int x, int yrow, const float *dc,
+__device__ __forceinline__ float4 grad_color(const GradDev &G, uint32_t c, int x0, int w, int x, int yrow, const float *dc,
                              const float *da) {
     const uint32_t gi = c - 1u;
     const float4 w0 = G.table[3 * gi], w1 = G.table[3 * gi + 1], w2 = G.table[3 * gi + 2];
@@ -1012,306 +1012,9 @@ __device__ float4 grad_color(const GradDev &G, uint32_t c, int x0, int w, int x,
     return o;
 }
 
-// ---------------------------------------------------------------------------------
-// compositor: one wavefront per 64x2 pixel tile; lane = pixel column, two rows.  For each
-// bucket of the tile's cell row (path order), the lanes blend the item covering their
-// pixels: src-over into an 8-bit framebuffer re-quantised after each blend, coverage
-// alpha = paint alpha * popcount(mask)/N (output_{8,32}.frag.glsl), or per-sample
-// blending + resolve in multisample mode (ms_output_{8,32}.frag.glsl).
-// ---------------------------------------------------------------------------------
-struct CompArgs {
-    int W, H, tiles_x;
-    const uint32_t *rowbeg, *rowend, *rowvals;
-    const int2 *binfo;
-    const uint32_t *finfo, *bicnt, *boff, *bext, *items, *imask, *icov;
-    const float *dec_c, *thr_c, *dec_a, *thr_a;
-    const uint8_t *lut_c, *lut_a;
-    uint8_t *image;
-    int pitch;
-    GradDev G;
-};
-
-// thr_* hold 257 entries: thr[256] = +inf ends the threshold bump at code 255.
-struct LdsTables {
-    float dec_c[256], thr_c[260], dec_a[256], thr_a[260];
-    uint8_t lut_c[4096], lut_a[4096];
-};
-
-// 8-bit code of a blended value: largest k with v >= thr[k] (thr[0] = -inf).  At most one
-// threshold lies in each 1/4096 interval, so the LUT entry of v's interval is either the
-// code or one below it.  Branch-free; negative values and NaN give 0, v >= 1 gives 255.
-__device__ __forceinline__ uint32_t enc8(const float *thr, const uint8_t *lut, float v) {
-    const float fi = v * 4096.f;
-    const int i = (fi >= 0.f) ? ((fi < 4095.f) ? (int)fi : 4095) : 0;
-    const uint32_t k = lut[i];
-    return k + ((v >= thr[k + 1]) ? 1u : 0u);
-}
-
-__device__ __forceinline__ uint32_t blend_px(const LdsTables &T, uint32_t dst, const float *sc, float a) {
-    const float oma = 1.f - a;
-    uint32_t out = 0u;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        float o = sc[ch] * a + T.dec_c[(dst >> (8 * ch)) & 0xFFu] * oma;
-        out |= enc8(T.thr_c, T.lut_c, o) << (8 * ch);
-    }
-    float oa = a * a + T.dec_a[dst >> 24] * oma;
-    out |= enc8(T.thr_a, T.lut_a, oa) << 24;
-    return out;
-}
-
-template <int SPP>
-__device__ __forceinline__ uint32_t pixel_bits(const uint32_t *m, int dx, int dy) {
-    if (SPP == 8) return (m[0] >> ((dx * 2 + dy) * 8)) & 0xFFu;
-    if (SPP == 16) return (m[dx] >> (dy * 16)) & 0xFFFFu;
-    return m[dx * 2 + dy];
-}
-
-constexpr int kListCap = 256;  // per-wave list of overlapping buckets
-constexpr int kGroup = 8;      // buckets whose items are loaded together
-
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
-}
-
-// Blend one bucket's coverage of this lane's two pixels with paints p0 / p1 (decoded
-// colour and alpha).  `bits0/bits1` are the sample masks (multisample mode) and `n0/n1`
-// the sample counts (coverage mode).
-template <int SPP, bool MS>
-__device__ __forceinline__ void blend_cov(const LdsTables &T, uint32_t *st0, uint32_t *st1, const float4 &p0,
-                                          const float4 &p1, uint32_t n0, uint32_t n1, uint32_t bits0, uint32_t bits1) {
-    constexpr int NS = MS ? SPP : 1;
-    const float sc0[3] = {p0.x, p0.y, p0.z}, sc1[3] = {p1.x, p1.y, p1.z};
-    if (!MS) {
-        st0[0] = blend_px(T, st0[0], sc0, p0.w * ((float)n0 / (float)SPP));
-        st1[0] = blend_px(T, st1[0], sc1, p1.w * ((float)n1 / (float)SPP));
-    } else {
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            if ((bits0 >> s) & 1u) st0[s] = blend_px(T, st0[s], sc0, p0.w);
-            if ((bits1 >> s) & 1u) st1[s] = blend_px(T, st1[s], sc1, p1.w);
-        }
-    }
-}
-
-__device__ __forceinline__ float4 solid_paint(const LdsTables &T, uint32_t c) {
-    return make_float4(T.dec_c[c & 0xFFu], T.dec_c[(c >> 8) & 0xFFu], T.dec_c[(c >> 16) & 0xFFu], T.dec_a[c >> 24]);
-}
-
-// The paints of fill word c at this lane's two pixels (column x of the item at px0 of
-// width pw, cell row r); false when the word paints nothing.
-template <bool GRAD>
-__device__ __forceinline__ bool item_paint(const CompArgs &A, const LdsTables &T, uint32_t c, int px0, int pw, int x,
-                                           int r, float4 &p0, float4 &p1) {
-    if (GRAD && !(c & 0xFF000000u)) {
-        if (!grad_paints(A.G, c)) return false;
-        const int w = pw ? pw : 2;
-        p0 = grad_color(A.G, c, px0, w, x, 2 * r, T.dec_c, T.dec_a);
-        p1 = grad_color(A.G, c, px0, w, x, 2 * r + 1, T.dec_c, T.dec_a);
-        return true;
-    }
-    p0 = p1 = solid_paint(T, c);
-    return true;
-}
-
-// Items of one bucket are disjoint and sorted by x: the covering item of column x is the
-// last one with x0 <= x, found by a lane-parallel binary search over the 64 loaded items.
-__device__ __forceinline__ int find_item(uint32_t it, int m, int x) {
-    const int ix0 = (int)(it & 0xFFFFu);
-    int pos = -1;
-#pragma unroll
-    for (int step = 64; step >= 1; step >>= 1) {
-        const int cand = pos + step;
-        const int cx0 = __shfl(ix0, cand < 64 ? cand : 63, 64);
-        if (cand < m && cx0 <= x) pos = cand;
-    }
-    return pos;
-}
-
-template <int SPP, bool MS, bool GRAD>
-__device__ __noinline__ void blend_bucket_chunked(const CompArgs &A, const LdsTables &T, uint32_t *st0, uint32_t *st1,
-                                                     uint32_t c, int cnt, uint32_t ib, int tx0, int x, int r) {
-    constexpr int NW = Words<SPP>::n;
-    const int lane = threadIdx.x & 63;
-    for (int ic = 0; ic < cnt; ic += 64) {
-        const int m = tmin(64, cnt - ic);
-        const uint32_t it = lane < m ? A.items[ib + ic + lane] : 0u;
-        const int ix0 = (int)(it & 0xFFFFu), iw = (int)(it >> 16);
-        const int iend = ix0 + (iw ? iw : 2);
-        if (__shfl(iend, m - 1, 64) <= tx0) continue;
-        if (__shfl(ix0, 0, 64) >= tx0 + 64) break;
-        const int pos = find_item(it, m, x);
-        const int p = pos < 0 ? 0 : pos;
-        const int pend = __shfl(iend, p, 64), pw = __shfl(iw, p, 64), px0 = __shfl(ix0, p, 64);
-        if (!(pos >= 0 && x < pend && x < A.W)) continue;
-        uint32_t n0 = SPP, n1 = SPP, b0 = 0xFFFFFFFFu, b1 = 0xFFFFFFFFu;
-        if (pw == 0) {
-            if (!MS) {
-                const uint32_t cv = A.icov[ib + ic + pos];
-                n0 = (cv >> (6 * ((x - px0) * 2))) & 63u;
-                n1 = (cv >> (6 * ((x - px0) * 2 + 1))) & 63u;
-            } else {
-                uint32_t mk[NW];
-                load_mask<NW>(A.imask, (size_t)ib + ic + pos, mk);
-                b0 = pixel_bits<SPP>(mk, x - px0, 0);
-                b1 = pixel_bits<SPP>(mk, x - px0, 1);
-            }
-        }
-        float4 p0, p1;
-        if (item_paint<GRAD>(A, T, c, px0, pw, x, r, p0, p1)) blend_cov<SPP, MS>(T, st0, st1, p0, p1, n0, n1, b0, b1);
-    }
-}
-
-// One wavefront per 64x2 pixel tile (lane = pixel column, rows 2r and 2r+1).  The row's
-// non-empty buckets are in path order; the wave first filters those overlapping the tile
-// into an LDS list (colour, item count, item base), then walks the list in groups of
-// kGroup buckets: it issues all the group's item loads together, and per bucket (in path
-// order) finds each column's covering item and blends it.  Blending is src-over into an
-// 8-bit framebuffer re-quantised after every blend: coverage alpha = paint alpha *
-// popcount(mask) / N (output_{8,32}.frag.glsl), or per-sample blending plus a resolve in
-// multisample mode (ms_output_{8,32}.frag.glsl).
-template <int SPP, bool MS, bool GRAD>
-__global__ void __launch_bounds__(256) k_composite(CompArgs A) {
-    constexpr int NW = Words<SPP>::n;
-    constexpr uint32_t FULLB = SPP == 32 ? 0xFFFFFFFFu : ((1u << SPP) - 1u);
-    __shared__ LdsTables T;
-    __shared__ uint32_t lcol[4][kListCap], lcnt[4][kListCap], lib[4][kListCap];
-    __shared__ uint32_t gitem[4][kGroup][64], gcv[4][kGroup][MS ? 1 : 64];
-    for (int i = threadIdx.x; i < 256; i += 256) {
-        T.dec_c[i] = A.dec_c[i]; T.thr_c[i] = A.thr_c[i];
-        T.dec_a[i] = A.dec_a[i]; T.thr_a[i] = A.thr_a[i];
-    }
-    if (threadIdx.x < 4) {
-        T.thr_c[256 + threadIdx.x] = __builtin_inff();
-        T.thr_a[256 + threadIdx.x] = __builtin_inff();
-    }
-    for (int i = threadIdx.x; i < 4096 / 4; i += 256) {
-        ((uint32_t *)T.lut_c)[i] = ((const uint32_t *)A.lut_c)[i];
-        ((uint32_t *)T.lut_a)[i] = ((const uint32_t *)A.lut_a)[i];
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int tile = blockIdx.x * 4 + w;
-    const int R = (A.H + 1) >> 1;
-    if (tile >= R * A.tiles_x) return;
-    const int r = tile / A.tiles_x;
-    const int tx0 = (tile - r * A.tiles_x) * 64;
-    const int tx1 = tmin(tx0 + 64, A.W);
-    const int x = tx0 + lane;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    constexpr int NS = MS ? SPP : 1;
-    uint32_t st0[NS], st1[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) { st0[s] = 0xFFFFFFFFu; st1[s] = 0xFFFFFFFFu; }
-
-    auto flush = [&](int nlist) {
-        wave_sync_lds();
-        for (int g0 = 0; g0 < nlist; g0 += kGroup) {
-            const int ng = tmin(kGroup, nlist - g0);
-            // issue the group's item loads together, park them in LDS
-#pragma unroll
-            for (int g = 0; g < kGroup; ++g) {
-                if (g < ng) {
-                    const int cnt = (int)lcnt[w][g0 + g];
-                    const uint32_t ib = lib[w][g0 + g];
-                    const bool ld = cnt <= 64 && lane < cnt;
-                    gitem[w][g][lane] = ld ? A.items[ib + lane] : 0u;
-                    if (!MS) gcv[w][g][lane] = ld ? A.icov[ib + lane] : 0u;
-                }
-            }
-            wave_sync_lds();
-            for (int g = 0; g < ng; ++g) {
-                const uint32_t c = lcol[w][g0 + g];
-                const int cnt = (int)lcnt[w][g0 + g];
-                const uint32_t ib = lib[w][g0 + g];
-                if (cnt > 64) {
-                    blend_bucket_chunked<SPP, MS, GRAD>(A, T, st0, st1, c, cnt, ib, tx0, x, r);
-                    continue;
-                }
-                const uint32_t it = gitem[w][g][lane];
-                const int pos = find_item(it, cnt, x);
-                const int p = pos < 0 ? 0 : pos;
-                const int ix0 = (int)(it & 0xFFFFu), iw = (int)(it >> 16);
-                const int iend = ix0 + (iw ? iw : 2);
-                const int pend = __shfl(iend, p, 64), pw = __shfl(iw, p, 64), px0 = __shfl(ix0, p, 64);
-                if (!(pos >= 0 && x < pend && x < A.W)) continue;
-                uint32_t n0 = SPP, n1 = SPP, b0 = FULLB, b1 = FULLB;
-                if (pw == 0) {
-                    if (!MS) {
-                        const uint32_t pcv = gcv[w][g][p];
-                        n0 = (pcv >> (6 * ((x - px0) * 2))) & 63u;
-                        n1 = (pcv >> (6 * ((x - px0) * 2 + 1))) & 63u;
-                    } else {
-                        uint32_t mk[NW];
-                        load_mask<NW>(A.imask, (size_t)ib + pos, mk);
-                        b0 = pixel_bits<SPP>(mk, x - px0, 0);
-                        b1 = pixel_bits<SPP>(mk, x - px0, 1);
-                    }
-                }
-                float4 p0, p1;
-                if (item_paint<GRAD>(A, T, c, px0, pw, x, r, p0, p1))
-                    blend_cov<SPP, MS>(T, st0, st1, p0, p1, n0, n1, b0, b1);
-            }
-            wave_sync_lds();
-        }
-    };
-
-    const uint32_t q0 = A.rowbeg[r], q1 = A.rowend[r];
-    int nlist = 0;
-    for (uint32_t qc = q0; qc < q1; qc += 64) {
-        const uint32_t q = qc + lane;
-        const bool inq = q < q1;
-        const uint32_t bb = inq ? A.rowvals[q] : 0u;
-        const uint32_t ext = inq ? A.bext[bb] : 0xFFFFu;
-        const int exlo = (int)(ext & 0xFFFFu), exhi = (int)(ext >> 16);
-        const uint32_t fc = inq ? A.finfo[A.binfo[bb].x] : 0u;
-        const bool paints = fc != 0u && (!GRAD || (fc & 0xFF000000u) || grad_paints(A.G, fc));
-        const bool ov = inq && exlo < tx1 && exhi > tx0 && paints;
-        const unsigned long long om = __ballot(ov);
-        if (!om) continue;
-        if (nlist + 64 > kListCap) {
-            flush(nlist);
-            nlist = 0;
-        }
-        if (ov) {
-            const int slot = nlist + (int)__popcll(om & lt);
-            lcol[w][slot] = fc;
-            lcnt[w][slot] = A.bicnt[bb];
-            lib[w][slot] = 2u * A.boff[bb];
-        }
-        nlist += (int)__popcll(om);
-    }
-    if (nlist) flush(nlist);
-
-    if (x >= A.W) return;
-    uint32_t o0, o1;
-    if (!MS) {
-        o0 = st0[0];
-        o1 = st1[0];
-    } else {
-        uint32_t outs[2];
-#pragma unroll
-        for (int rr = 0; rr < 2; ++rr) {
-            const uint32_t *stp = rr ? st1 : st0;
-            uint32_t o = 0u;
-#pragma unroll
-            for (int ch = 0; ch < 4; ++ch) {
-                const float *dec = ch < 3 ? T.dec_c : T.dec_a;
-                float sum = 0.f;
-#pragma unroll
-                for (int s = 0; s < NS; ++s) sum += dec[(stp[s] >> (8 * ch)) & 0xFFu];
-                const float avg = sum / (float)SPP;
-                o |= (ch < 3 ? enc8(T.thr_c, T.lut_c, avg) : enc8(T.thr_a, T.lut_a, avg)) << (8 * ch);
-            }
-            outs[rr] = o;
-        }
-        o0 = outs[0];
-        o1 = outs[1];
-    }
-    const int y0 = 2 * r;
-    *(uint32_t *)(A.image + (size_t)y0 * A.pitch + (size_t)x * 4) = o0;
-    if (y0 + 1 < A.H) *(uint32_t *)(A.image + (size_t)(y0 + 1) * A.pitch + (size_t)x * 4) = o1;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1385,6 +1088,9 @@ struct CovArgs {
     const uint4 *rrec;          // per row-list entry: extent, colour, item count, item base
     const uint64_t *rtile;      // per row-list entry: 64-pixel tiles holding items
     const uint32_t *items, *icov;
+    const uint32_t *imask;      // merged masks (multisample output)
+    const uint32_t *rpath;      // row-list path ids (multisample output)
+    const uint32_t *ptab;       // per-path blend tables (multisample output)
     const float *dec_c;         // [256] colour decode, for the paint colours
     const float *dec_a;         // [256] alpha decode (gradient ramp texels)
     const float4 *step_c;       // [256] sRGB steps {thr[k+1], dec[k], dec[k+1], 0}
@@ -1612,17 +1318,425 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_comp
     if (y0 + 1 < H) *(uint32_t *)(image + (size_t)(y0 + 1) * pitch + (size_t)x * 4) = pack_dec<SRGB>(s_step, s_lut, d1);
 }
 
-// Row-list records for the coverage compositor: one 16-byte load per listed bucket.
+// ---------------------------------------------------------------------------------
+// multisample compositor (ms_output_{8,32}.frag.glsl: every covered sample of an N-sample
+// 8-bit framebuffer is blended src-over and re-quantised; ras_base.cpp:1148-1163 then
+// resolves each pixel to the average of its samples' decoded values, summed in sample
+// order).  The bucket lookup is the coverage compositor's (lane = pixel column); the
+// blending runs with lane = sample: lane q * 2N + dy * N + s holds sample s of pixel
+// (column p * CPR + q, row 2r + dy) of the 64x2 tile in state register p, CPR = 64 / 2N
+// columns per register.  Each register that holds a pixel covered by the bucket costs one
+// wave-wide blend (its lanes select by their sample bits), so a 2x2 fragment or a span
+// column costs the same whatever its sample count, and no lane loops over samples.  The
+// register index is wave-uniform, so the indexed state stays in VGPRs (s_set_gpr_idx).
+// ---------------------------------------------------------------------------------
+typedef uint32_t u32x32 __attribute__((ext_vector_type(32)));
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
+
+// Runs f(state, p) -> new state on every register p of the set bits of m (p = pbase + bit).
+// The 4-way form takes four registers an iteration so that their LDS round trips overlap;
+// missing ones repeat the first register, whose duplicate writes store the same value, so
+// every write is unconditional.  A write that depends on control flow inside the loop
+// makes the compiler copy the whole 32-register vector at every iteration, and so does a
+// loop that writes two vectors: each call covers one.
+template <class V, class F>
+__device__ __forceinline__ void apply_regs4(V &vec, uint32_t m, int pbase, const F &f) {
+    while (m) {
+        const int c0 = __builtin_amdgcn_readfirstlane((int)__builtin_ctz(m));
+        m &= m - 1u;
+        const int c1 = m ? __builtin_amdgcn_readfirstlane((int)__builtin_ctz(m)) : c0;
+        m &= m - 1u;
+        const int c2 = m ? __builtin_amdgcn_readfirstlane((int)__builtin_ctz(m)) : c0;
+        m &= m - 1u;
+        const int c3 = m ? __builtin_amdgcn_readfirstlane((int)__builtin_ctz(m)) : c0;
+        m &= m - 1u;
+        const uint32_t o0 = vec[c0], o1 = vec[c1], o2 = vec[c2], o3 = vec[c3];
+        const uint32_t n0 = f(o0, pbase + c0), n1 = f(o1, pbase + c1), n2 = f(o2, pbase + c2),
+                       n3 = f(o3, pbase + c3);
+        vec[c3] = n3;
+        vec[c2] = n2;
+        vec[c1] = n1;
+        vec[c0] = n0;
+    }
+}
+
+template <class V, class F>
+__device__ __forceinline__ void apply_regs1(V &vec, uint32_t m, int pbase, const F &f) {
+    while (m) {
+        const int c = __builtin_amdgcn_readfirstlane((int)__builtin_ctz(m));
+        m &= m - 1u;
+        vec[c] = f(vec[c], pbase + c);
+    }
+}
+
+template <int NREG> struct SampleRegs;
+template <> struct SampleRegs<64> {
+    u32x32 a, b;
+    __device__ __forceinline__ void init(uint32_t v) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) { a[i] = v; b[i] = v; }
+    }
+    __device__ __forceinline__ uint32_t get(int p) const { return p < 32 ? a[p] : b[p - 32]; }
+    template <class F> __device__ __forceinline__ void apply4(uint64_t m, const F &f) {
+        apply_regs4(a, (uint32_t)m, 0, f);
+        apply_regs4(b, (uint32_t)(m >> 32), 32, f);
+    }
+    template <class F> __device__ __forceinline__ void apply1(uint64_t m, const F &f) {
+        apply_regs1(a, (uint32_t)m, 0, f);
+        apply_regs1(b, (uint32_t)(m >> 32), 32, f);
+    }
+};
+template <> struct SampleRegs<32> {
+    u32x32 a;
+    __device__ __forceinline__ void init(uint32_t v) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) a[i] = v;
+    }
+    __device__ __forceinline__ uint32_t get(int p) const { return a[p]; }
+    template <class F> __device__ __forceinline__ void apply4(uint64_t m, const F &f) { apply_regs4(a, (uint32_t)m, 0, f); }
+    template <class F> __device__ __forceinline__ void apply1(uint64_t m, const F &f) { apply_regs1(a, (uint32_t)m, 0, f); }
+};
+template <> struct SampleRegs<16> {
+    u32x16 a;
+    __device__ __forceinline__ void init(uint32_t v) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) a[i] = v;
+    }
+    __device__ __forceinline__ uint32_t get(int p) const { return a[p]; }
+    template <class F> __device__ __forceinline__ void apply4(uint64_t m, const F &f) { apply_regs4(a, (uint32_t)m, 0, f); }
+    template <class F> __device__ __forceinline__ void apply1(uint64_t m, const F &f) { apply_regs1(a, (uint32_t)m, 0, f); }
+};
+
+// Blend tables of the solid paints, one per path, built once per frame: entry k holds, in
+// byte ch, the code that channel ch takes when the path's paint (decoded colour, alpha a)
+// is blended over code k -- blend_code's arithmetic per channel.  Lane l of a path's wave
+// writes entries 4l .. 4l + 3, the 16 bytes the compositor's lane l loads.  Words without
+// alpha (gradients, no paint) get no table.
+template <bool SRGB>
+__global__ void __launch_bounds__(256) k_paint_tables(uint32_t P, const uint32_t *__restrict__ finfo,
+                                                      const float *__restrict__ dec, const float4 *__restrict__ step,
+                                                      const uint8_t *__restrict__ lut, uint32_t *__restrict__ ptab) {
+    const uint32_t path = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (path >= P) return;
+    const uint32_t c = finfo[path];
+    if (!(c & 0xFF000000u)) return;
+    const int lane = threadIdx.x & 63;
+    const float4 src = make_float4(dec[c & 0xFFu], dec[(c >> 8) & 0xFFu], dec[(c >> 16) & 0xFFu], unorm_dec((int)(c >> 24)));
+    const float a = src.w, oma = 1.f - a;
+    uint32_t e[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int k = 4 * lane + j;
+        const float dk = SRGB ? dec[k] : unorm_dec(k);
+        const float v0 = src.x * a + dk * oma;
+        const float v1 = src.y * a + dk * oma;
+        const float v2 = src.z * a + dk * oma;
+        const float va = a * a + unorm_dec(k) * oma;
+        float t;
+        const uint32_t c0 = SRGB ? (uint32_t)srgb_code(step, lut, v0, t) : (uint32_t)unorm_code(v0);
+        const uint32_t c1 = SRGB ? (uint32_t)srgb_code(step, lut, v1, t) : (uint32_t)unorm_code(v1);
+        const uint32_t c2 = SRGB ? (uint32_t)srgb_code(step, lut, v2, t) : (uint32_t)unorm_code(v2);
+        e[j] = c0 | (c1 << 8) | (c2 << 16) | ((uint32_t)unorm_code(va) << 24);
+    }
+    *(uint4 *)(ptab + (size_t)path * 256 + 4 * lane) = make_uint4(e[0], e[1], e[2], e[3]);
+}
+
+__device__ __forceinline__ uint32_t table_code(const uint32_t *T, uint32_t st) {
+    const uint8_t *b = (const uint8_t *)T;
+    return (uint32_t)b[(st & 0xFFu) * 4] | ((uint32_t)b[((st >> 8) & 0xFFu) * 4 + 1] << 8) |
+           ((uint32_t)b[((st >> 16) & 0xFFu) * 4 + 2] << 16) | ((uint32_t)b[(st >> 24) * 4 + 3] << 24);
+}
+
+// One sample's framebuffer code blended with paint src (decoded colour, alpha a) and
+// re-quantised: the arithmetic of blend_dec on the decoded code.
+template <bool SRGB>
+__device__ __forceinline__ uint32_t blend_code(const float4 *step, const uint8_t *lut, const float *dec, uint32_t code,
+                                               const float4 &src, float a) {
+    const float oma = 1.f - a;
+    const uint32_t k0 = code & 0xFFu, k1 = (code >> 8) & 0xFFu, k2 = (code >> 16) & 0xFFu;
+    const float d0 = SRGB ? dec[k0] : unorm_dec((int)k0);
+    const float d1 = SRGB ? dec[k1] : unorm_dec((int)k1);
+    const float d2 = SRGB ? dec[k2] : unorm_dec((int)k2);
+    const float da = unorm_dec((int)(code >> 24));
+    const float v0 = src.x * a + d0 * oma;
+    const float v1 = src.y * a + d1 * oma;
+    const float v2 = src.z * a + d2 * oma;
+    const float va = a * a + da * oma;
+    float t;
+    const uint32_t c0 = SRGB ? (uint32_t)srgb_code(step, lut, v0, t) : (uint32_t)unorm_code(v0);
+    const uint32_t c1 = SRGB ? (uint32_t)srgb_code(step, lut, v1, t) : (uint32_t)unorm_code(v1);
+    const uint32_t c2 = SRGB ? (uint32_t)srgb_code(step, lut, v2, t) : (uint32_t)unorm_code(v2);
+    return c0 | (c1 << 8) | (c2 << 16) | ((uint32_t)unorm_code(va) << 24);
+}
+
+// Bit i of the result is set when any of bits [i * CPR, (i + 1) * CPR) of e is.
+template <int CPR>
+__device__ __forceinline__ uint64_t group_bits(uint64_t e) {
+    if (CPR == 1) return e;
+    if (CPR == 2) {
+        uint64_t x = (e | (e >> 1)) & 0x5555555555555555ull;
+        x = (x | (x >> 1)) & 0x3333333333333333ull;
+        x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+        x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+        x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+        return (x | (x >> 16)) & 0xFFFFFFFFull;
+    }
+    uint64_t x = (e | (e >> 1) | (e >> 2) | (e >> 3)) & 0x1111111111111111ull;
+    x = (x | (x >> 3)) & 0x0303030303030303ull;
+    x = (x | (x >> 6)) & 0x000F000F000F000Full;
+    x = (x | (x >> 12)) & 0x000000FF000000FFull;
+    return (x | (x >> 24)) & 0xFFFFull;
+}
+
+// The sample bits of the two pixels of column offset dx in a merged mask (the reference's
+// layout: N = 8 one byte per pixel (dx * 2 + dy), N = 16 one half-word per pixel in word dx,
+// N = 32 one word per pixel dx * 2 + dy): row 0 in the low N bits, row 1 above (N = 32:
+// row 1 in `hi`).
+template <int SPP>
+__device__ __forceinline__ uint32_t column_bits(const uint32_t *m, int dx, uint32_t &hi) {
+    if (SPP == 32) {
+        hi = dx ? m[3] : m[1];
+        return dx ? m[2] : m[0];
+    }
+    hi = 0u;
+    if (SPP == 16) return dx ? m[1] : m[0];
+    return (m[0] >> (16 * dx)) & 0xFFFFu;
+}
+
+template <int SPP, bool SRGB, bool GRAD>
+__global__ void __launch_bounds__(256) k_composite_ms(CovArgs A) {
+    constexpr int NW = Words<SPP>::n;
+    constexpr int CPR = 64 / (2 * SPP);
+    constexpr int NREG = 64 / CPR;
+    constexpr int G = 4;
+    constexpr uint32_t FULL = SPP == 8 ? 0xFFFFu : 0xFFFFFFFFu;  // both rows' bits (N = 32: each row)
+    __shared__ float4 s_step[SRGB ? 256 : 1];
+    __shared__ uint8_t s_lut[SRGB ? 4096 : 4];
+    __shared__ float s_dec[256];
+    __shared__ float s_deca[GRAD ? 256 : 1];
+    __shared__ float4 l_src[4][kCovList];
+    __shared__ uint2 l_ref[4][kCovList];
+    __shared__ uint32_t l_pid[4][kCovList];
+    __shared__ uint32_t slot[4][G][64];
+    __shared__ uint32_t tbl[4][G][256];
+    for (int i = threadIdx.x; i < 256; i += 256) {
+        if (SRGB) s_step[i] = A.step_c[i];
+        s_dec[i] = A.dec_c[i];
+        if (GRAD) s_deca[i] = A.dec_a[i];
+    }
+    for (int i = threadIdx.x; i < (SRGB ? 1024 : 0); i += 256) ((uint32_t *)s_lut)[i] = ((const uint32_t *)A.lut_c)[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int W = A.W, H = A.H, tiles_x = A.tiles_x;
+    // XCD-aware tile order of the coverage compositor: XCD b % 8 takes whole cell rows
+    const int xcd = (int)(blockIdx.x & 7u), t = (int)(blockIdx.x >> 3) * 4 + w;
+    const int rr = t / tiles_x, r = xcd + 8 * rr;
+    if (r >= (H + 1) / 2) return;
+    const int tx0 = (t - rr * tiles_x) * 64;
+    const int tx1 = tmin(tx0 + 64, W);
+    const int x = tx0 + lane;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // blend-phase coordinates of this lane
+    const int q = lane / (2 * SPP), dy = (lane / SPP) & 1, smp = lane & (SPP - 1);
+    SampleRegs<NREG> S;
+    S.init(0xFFFFFFFFu);
+
+    auto flush = [&](int nlist) {
+        wave_sync_lds();
+        for (int g0 = 0; g0 < nlist; g0 += G) {
+            const int ng = tmin(G, nlist - g0);
+            uint32_t it[G], mk[G][NW];
+            uint4 tv[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                slot[w][g][lane] = 0u;
+                const uint2 ref = g < ng ? l_ref[w][g0 + g] : make_uint2(0u, 0u);
+                const bool ld = lane < (int)ref.y;
+                it[g] = ld ? A.items[ref.x + lane] : 0u;
+#pragma unroll
+                for (int k = 0; k < NW; ++k) mk[g][k] = 0u;
+                if (ld) load_mask<NW>(A.imask, (size_t)ref.x + lane, mk[g]);
+                // the path's blend table (solid paints; a gradient's l_src.w is negative)
+                const bool solid = g < ng && !(GRAD && l_src[w][g0 + g].w < 0.f);
+                tv[g] = solid ? *(const uint4 *)(A.ptab + (size_t)l_pid[w][g0 + g] * 256 + 4 * lane)
+                              : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) *(uint4 *)&tbl[w][g][4 * lane] = tv[g];
+            wave_sync_lds();
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (g >= ng) break;
+                const uint2 ref = l_ref[w][g0 + g];
+                if (ref.y > 64u) {
+                    scatter_big(A.items, ref.x, (int)ref.y, tx0, slot[w][g]);
+                } else if (lane < (int)ref.y) {
+                    const int x0 = (int)(it[g] & 0xFFFFu), iw = (int)(it[g] >> 16);
+                    const int end = x0 + (iw ? iw : 2);
+                    if (end > tx0 && x0 < tx0 + 64) atomicMax(&slot[w][g][tmax(x0 - tx0, 0)], (uint32_t)(lane + 1));
+                }
+            }
+            wave_sync_lds();
+            // per column (lane): the sample bits of both rows under each bucket's covering
+            // item, 0 where no item covers it; the covering item's extent for gradients
+            uint32_t cb[G], cbh[G], ce[GRAD ? G : 1];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                cb[g] = 0u;
+                cbh[g] = 0u;
+                if (GRAD) ce[g] = 0u;
+                if (g >= ng) break;
+                const uint2 ref = l_ref[w][g0 + g];
+                const int p = (int)(uint32_t)dpp_scan((int)slot[w][g][lane], 0, OpMax()) - 1;
+                uint32_t pit, pm[NW];
+                if (ref.y > 64u) {
+                    pit = p >= 0 ? A.items[ref.x + p] : 0u;
+#pragma unroll
+                    for (int k = 0; k < NW; ++k) pm[k] = 0u;
+                    if (p >= 0) load_mask<NW>(A.imask, (size_t)ref.x + p, pm);
+                } else {
+                    pit = (uint32_t)__shfl((int)it[g], tmax(p, 0), 64);
+#pragma unroll
+                    for (int k = 0; k < NW; ++k) pm[k] = (uint32_t)__shfl((int)mk[g][k], tmax(p, 0), 64);
+                }
+                const int px0 = (int)(pit & 0xFFFFu), pw = (int)(pit >> 16);
+                const int pend = px0 + (pw ? pw : 2);
+                if (p >= 0 && x < pend && x < W) {
+                    uint32_t hi = FULL, lo = FULL;
+                    if (pw == 0) lo = column_bits<SPP>(pm, x - px0, hi);
+                    cb[g] = lo;
+                    cbh[g] = SPP == 32 ? hi : 0u;
+                    if (GRAD) ce[g] = (uint32_t)px0 | ((uint32_t)(pw ? pw : 2) << 16);
+                }
+            }
+            // blend, bucket by bucket in path order, one wave-wide pass per state register
+            // holding a covered column: four LDS reads of the path's blend table for a
+            // solid paint, the blend arithmetic on the pixel's own paint for a gradient.
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (g >= ng) break;
+                const uint64_t regs = group_bits<CPR>(__ballot((cb[g] | cbh[g]) != 0u));
+                if (!regs) continue;
+                const float4 src = l_src[w][g0 + g];
+                const bool gpaint = GRAD && src.w < 0.f;
+                const uint32_t cbg = cb[g], cbhg = cbh[g], ceg = GRAD ? ce[g] : 0u;
+                // this lane's sample bit under register p's covering item
+                auto covers = [&](int p) -> bool {
+                    uint32_t bits;
+                    if (SPP == 32) {
+                        const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)cbg, p);
+                        const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)cbhg, p);
+                        bits = dy ? b1 : b0;
+                    } else {
+                        bits = (uint32_t)__shfl((int)cbg, p * CPR + q, 64) >> (dy * SPP);
+                    }
+                    return (bits >> smp) & 1u;
+                };
+                if (!gpaint) {
+                    S.apply4(regs, [&](uint32_t st, int p) -> uint32_t {
+                        const uint32_t nst = table_code(tbl[w][g], st);
+                        return covers(p) ? nst : st;
+                    });
+                } else {
+                    S.apply1(regs, [&](uint32_t st, int p) -> uint32_t {
+                        const int c = p * CPR + q;
+                        const uint32_t e = (uint32_t)__shfl((int)ceg, c, 64);
+                        const float4 pa = grad_color(A.G, __float_as_uint(src.x), (int)(e & 0xFFFFu), (int)(e >> 16),
+                                                     tx0 + c, 2 * r + dy, s_dec, s_deca);
+                        const uint32_t nst = blend_code<SRGB>(s_step, s_lut, s_dec, st, pa, pa.w);
+                        return covers(p) ? nst : st;
+                    });
+                }
+            }
+            wave_sync_lds();
+        }
+    };
+
+    const uint32_t q0 = A.rowbeg[r], q1 = A.rowend[r];
+    const int tbit = (tx0 >> 6) & 63;
+    int nlist = 0;
+    for (uint32_t qc = q0; qc < q1; qc += 64) {
+        const bool inq = qc + lane < q1;
+        const uint4 rec = inq ? A.rrec[qc + lane] : make_uint4(0xFFFFu, 0u, 0u, 0u);
+        const uint64_t rt = inq ? A.rtile[qc + lane] : 0ull;
+        const int exlo = (int)(rec.x & 0xFFFFu), exhi = (int)(rec.x >> 16);
+        const bool paints = (rec.y & 0xFF000000u) || (GRAD && rec.y != 0u && grad_paints(A.G, rec.y));
+        const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tbit) & 1ull) && paints;
+        const unsigned long long om = __ballot(ov);
+        if (!om) continue;
+        const int nov = (int)__popcll(om);
+        if (nlist + nov > kCovList) {
+            flush(nlist);
+            nlist = 0;
+        }
+        if (ov) {
+            const int sl = nlist + (int)__popcll(om & lt);
+            const uint32_t c = rec.y;
+            if (GRAD && !(c & 0xFF000000u))
+                l_src[w][sl] = make_float4(__uint_as_float(c), 0.f, 0.f, -1.f);
+            else
+                l_src[w][sl] = make_float4(s_dec[c & 0xFFu], s_dec[(c >> 8) & 0xFFu], s_dec[(c >> 16) & 0xFFu],
+                                           unorm_dec((int)(c >> 24)));
+            l_ref[w][sl] = make_uint2(rec.w, rec.z);
+            l_pid[w][sl] = A.rpath[qc + lane];
+        }
+        nlist += nov;
+    }
+    if (nlist) flush(nlist);
+
+    // resolve: G registers at a time are staged in this wave's slot rows (G x 64 words);
+    // lane (pixel, channel) sums the pixel's decoded samples in sample order, divides by N
+    // and re-quantises, and the four channel lanes of a pixel assemble its word
+    static_assert(NREG % G == 0, "state registers are staged G at a time");
+    uint32_t *stg = &slot[w][0][0];
+    wave_sync_lds();
+#pragma unroll
+    for (int p0 = 0; p0 < NREG; p0 += G) {
+#pragma unroll
+        for (int k = 0; k < G; ++k) stg[k * 64 + lane] = S.get(p0 + k);
+        wave_sync_lds();
+#pragma unroll
+        for (int t0 = 0; t0 < 8 * G * CPR; t0 += 64) {
+            const int tt = t0 + lane;
+            const int ch = tt & 3, pix = tt >> 2;           // pix = (k * CPR + qq) * 2 + row, k < G
+            const int row = pix & 1, kq = pix >> 1, k = kq / CPR, qq = kq - k * CPR;
+            const bool live = tt < 8 * G * CPR;
+            const uint32_t *smps = stg + k * 64 + qq * 2 * SPP + row * SPP;
+            float sum = 0.f;
+            if (live) {
+#pragma unroll 8
+                for (int s2 = 0; s2 < SPP; ++s2) {
+                    const uint32_t b = (smps[s2] >> (8 * ch)) & 0xFFu;
+                    sum += (SRGB && ch < 3) ? s_dec[b] : unorm_dec((int)b);
+                }
+            }
+            const float avg = sum / (float)SPP;
+            float tmp;
+            const uint32_t code = (SRGB && ch < 3) ? (uint32_t)srgb_code(s_step, s_lut, avg, tmp) : (uint32_t)unorm_code(avg);
+            uint32_t word = code << (8 * ch);
+            word |= (uint32_t)__shfl_xor((int)word, 1, 64);
+            word |= (uint32_t)__shfl_xor((int)word, 2, 64);
+            const int col = tx0 + (p0 + k) * CPR + qq, y = 2 * r + row;
+            if (live && ch == 0 && col < W && y < H) *(uint32_t *)(A.image + (size_t)y * A.pitch + (size_t)col * 4) = word;
+        }
+        wave_sync_lds();
+    }
+}
+
+// Row-list records for the compositors: one 16-byte load per listed bucket (and its path
+// id for the multisample compositor's blend tables).
 __global__ void k_row_records(int64_t NB, const uint32_t *__restrict__ n_ptr, const uint32_t *__restrict__ rowvals,
                               const uint32_t *__restrict__ bext, const int2 *__restrict__ binfo,
                               const uint32_t *__restrict__ finfo, const uint32_t *__restrict__ bicnt,
                               const uint32_t *__restrict__ boff, const uint64_t *__restrict__ btile,
-                              uint4 *__restrict__ rrec, uint64_t *__restrict__ rtile) {
+                              uint4 *__restrict__ rrec, uint64_t *__restrict__ rtile, uint32_t *__restrict__ rpath) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= NB || q >= (int64_t)*n_ptr) return;
     const uint32_t bb = rowvals[q];
-    rrec[q] = make_uint4(bext[bb], finfo[binfo[bb].x], bicnt[bb], 2u * boff[bb]);
+    const uint32_t path = (uint32_t)binfo[bb].x;
+    rrec[q] = make_uint4(bext[bb], finfo[path], bicnt[bb], 2u * boff[bb]);
     rtile[q] = btile[bb];
+    if (rpath) rpath[q] = path;
 }
 
 // Device requantisation of arbitrary values (colour-table self-check): codes and the
@@ -1811,11 +1925,10 @@ void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_row_bounds, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, (const uint32_t *)f.nz_keys[0], n_ptr,
                        f.rowbeg, f.rowend);
-    if (!f.ms)
-        hipLaunchKernelGGL(k_row_records, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB, n_ptr,
+    hipLaunchKernelGGL(k_row_records, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB, n_ptr,
                            (const uint32_t *)f.nz_vals[0], (const uint32_t *)f.bext, (const int2 *)f.binfo, sc.finfo,
                            (const uint32_t *)f.bicnt, (const uint32_t *)f.boff, (const uint64_t *)f.btile, f.rrec,
-                           f.rtile);
+                           f.rtile, f.rpath);
 }
 
 void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s) {
@@ -1826,7 +1939,7 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
         CovArgs C;
         C.W = f.W; C.H = f.H; C.tiles_x = tiles_x;
         C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec; C.rtile = f.rtile;
-        C.items = f.items; C.icov = f.icov; C.dec_c = t.dec_c; C.dec_a = t.dec_a; C.step_c = t.step_c; C.lut_c = t.lut_c;
+        C.items = f.items; C.icov = f.icov; C.imask = nullptr; C.rpath = nullptr; C.ptab = nullptr; C.dec_c = t.dec_c; C.dec_a = t.dec_a; C.step_c = t.step_c; C.lut_c = t.lut_c;
         C.image = f.image; C.pitch = f.pitch;
         C.G = f.grad;
         C.nscenes = f.nscenes; C.ntiles = ntiles; C.stile = f.stile; C.simg = f.simg;
@@ -1856,24 +1969,37 @@ void launch_composite(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t
 #undef SL_LAUNCH
         return;
     }
-    CompArgs A;
-    A.W = f.W; A.H = f.H; A.tiles_x = (f.W + 63) / 64;
-    A.rowbeg = f.rowbeg; A.rowend = f.rowend; A.rowvals = f.nz_vals[0];
-    A.binfo = f.binfo; A.finfo = sc.finfo; A.bicnt = f.bicnt; A.boff = f.boff; A.bext = f.bext;
-    A.items = f.items; A.imask = f.imask; A.icov = f.icov;
-    A.dec_c = t.dec_c; A.thr_c = t.thr_c; A.dec_a = t.dec_a; A.thr_a = t.thr_a; A.lut_c = t.lut_c; A.lut_a = t.lut_a;
-    A.image = f.image; A.pitch = f.pitch;
-    A.G = f.grad;
-    dim3 g(blocks_for(ntiles, 4)), b(256);
-    if (f.has_grad) {
-        if (t.spp == 8) hipLaunchKernelGGL((k_composite<8, true, true>), g, b, 0, s, A);
-        else if (t.spp == 16) hipLaunchKernelGGL((k_composite<16, true, true>), g, b, 0, s, A);
-        else hipLaunchKernelGGL((k_composite<32, true, true>), g, b, 0, s, A);
-    } else {
-        if (t.spp == 8) hipLaunchKernelGGL((k_composite<8, true, false>), g, b, 0, s, A);
-        else if (t.spp == 16) hipLaunchKernelGGL((k_composite<16, true, false>), g, b, 0, s, A);
-        else hipLaunchKernelGGL((k_composite<32, true, false>), g, b, 0, s, A);
+    CovArgs C;
+    C.W = f.W; C.H = f.H; C.tiles_x = tiles_x;
+    C.rowbeg = f.rowbeg; C.rowend = f.rowend; C.rrec = f.rrec; C.rtile = f.rtile;
+    C.items = f.items; C.icov = f.icov; C.imask = f.imask; C.rpath = f.rpath; C.ptab = f.ptab;
+    C.dec_c = t.dec_c; C.dec_a = t.dec_a; C.step_c = t.step_c; C.lut_c = t.lut_c;
+    C.image = f.image; C.pitch = f.pitch;
+    C.G = f.grad;
+    C.nscenes = 0; C.ntiles = ntiles; C.stile = nullptr; C.simg = nullptr;
+    if (sc.P) {
+        if (f.srgb)
+            hipLaunchKernelGGL(k_paint_tables<true>, dim3(blocks_for(sc.P, 4)), dim3(256), 0, s, (uint32_t)sc.P, sc.finfo,
+                               t.dec_c, t.step_c, t.lut_c, f.ptab);
+        else
+            hipLaunchKernelGGL(k_paint_tables<false>, dim3(blocks_for(sc.P, 4)), dim3(256), 0, s, (uint32_t)sc.P, sc.finfo,
+                               t.dec_c, t.step_c, t.lut_c, f.ptab);
     }
+    dim3 g(8u * blocks_for((int64_t)tiles_x * ((R + 7) / 8), 4)), b(256);
+#define SL_LAUNCH(N)                                                                                    \
+    do {                                                                                                \
+        if (f.has_grad) {                                                                               \
+            if (f.srgb) hipLaunchKernelGGL((k_composite_ms<N, true, true>), g, b, 0, s, C);             \
+            else hipLaunchKernelGGL((k_composite_ms<N, false, true>), g, b, 0, s, C);                   \
+        } else {                                                                                        \
+            if (f.srgb) hipLaunchKernelGGL((k_composite_ms<N, true, false>), g, b, 0, s, C);            \
+            else hipLaunchKernelGGL((k_composite_ms<N, false, false>), g, b, 0, s, C);                  \
+        }                                                                                               \
+    } while (0)
+    if (t.spp == 8) SL_LAUNCH(8);
+    else if (t.spp == 16) SL_LAUNCH(16);
+    else SL_LAUNCH(32);
+#undef SL_LAUNCH
 }
 
 void launch_encode_values(const Tables &t, int channel, const float *v, int64_t n, uint8_t *code, float *dec,
