@@ -60,13 +60,16 @@ def stage_bytes(stage, cnt, spp, W, H, ms=False):
 
 
 # the kernel that makes up each stage's time (the stage's HIP-event interval)
-STAGE_KERNEL = {("composite", False): "k_composite_cov", ("composite", True): "k_composite",
+STAGE_KERNEL = {("composite", False): "k_composite_cov", ("composite", True): "k_composite_ms",
                 ("resolve", False): "k_resolve_first", ("resolve", True): "k_resolve_first",
                 ("fragments", False): "k_frag_scatter", ("fragments", True): "k_frag_scatter",
                 ("intersect", False): "k_chains", ("intersect", True): "k_chains"}
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plain_2048_32spp_traffic.md")
-INSTR_PROFILE = os.path.join(ROOT, "profiles", "plain_2048_32spp_instructions.md")
+# committed rocprofv3 summaries of the headline workload, per output mode
+TRAFFIC_PROFILE = {False: os.path.join(ROOT, "profiles", "plain_2048_32spp_traffic.md"),
+                   True: os.path.join(ROOT, "profiles", "plain_2048_32spp_ms_traffic.md")}
+INSTR_PROFILE = {False: os.path.join(ROOT, "profiles", "plain_2048_32spp_instructions.md"),
+                 True: os.path.join(ROOT, "profiles", "plain_2048_32spp_ms_instructions.md")}
 
 
 def _profile_value(path, key, kernel):
@@ -80,16 +83,16 @@ def _profile_value(path, key, kernel):
     return None
 
 
-def profiled_traffic(kernel):
+def profiled_traffic(kernel, ms=False):
     """HBM bytes per launch of `kernel` on the headline workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if absent."""
-    return _profile_value(TRAFFIC_PROFILE, "kernel_traffic_bytes", kernel)
+    return _profile_value(TRAFFIC_PROFILE[ms], "kernel_traffic_bytes", kernel)
 
 
-def profiled_valu(kernel):
+def profiled_valu(kernel, ms=False):
     """Wave VALU instructions per launch of `kernel` on the headline workload, from the
     committed rocprofv3 SQ_INSTS_VALU summary; None if absent."""
-    return _profile_value(INSTR_PROFILE, "kernel_valu_insts", kernel)
+    return _profile_value(INSTR_PROFILE[ms], "kernel_valu_insts", kernel)
 
 
 def b_alg_total(cnt, spp, W, H):
@@ -396,9 +399,9 @@ def main():
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
     dom_kernel = STAGE_KERNEL.get((dom, args.ms), dom)
     traffic = valu_frac = valu_ceiling = None
-    if args.config == 2 and not args.ms and (W, args.paths, args.spp) == (2048, 20000, 32):
-        traffic = profiled_traffic(dom_kernel)
-        valu = profiled_valu(dom_kernel)
+    if args.config == 2 and (W, args.paths, args.spp) == (2048, 20000, 32):
+        traffic = profiled_traffic(dom_kernel, args.ms)
+        valu = profiled_valu(dom_kernel, args.ms)
         if valu:
             # issued VALU instructions over what the SIMDs issued in the fma probe (measured
             # now, on this GPU and clock), scaled to the kernel's time

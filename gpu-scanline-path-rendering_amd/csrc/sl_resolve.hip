@@ -1407,10 +1407,10 @@ template <> struct SampleRegs<16> {
     template <class F> __device__ __forceinline__ void apply1(uint64_t m, const F &f) { apply_regs1(a, (uint32_t)m, 0, f); }
 };
 
-// Blend tables of the solid paints, one per path, built once per frame: entry k holds, in
-// byte ch, the code that channel ch takes when the path's paint (decoded colour, alpha a)
-// is blended over code k -- blend_code's arithmetic per channel.  Lane l of a path's wave
-// writes entries 4l .. 4l + 3, the 16 bytes the compositor's lane l loads.  Words without
+// Blend tables of the solid paints, one per path, built once per frame: byte ch * 256 + k
+// of a path's 1 KiB table holds the code that channel ch takes when the path's paint
+// (decoded colour, alpha a) is blended over code k -- blend_code's arithmetic per channel.
+// Lane l of a path's wave computes codes 4l .. 4l + 3 of every channel.  Words without
 // alpha (gradients, no paint) get no table.
 template <bool SRGB>
 __global__ void __launch_bounds__(256) k_paint_tables(uint32_t P, const uint32_t *__restrict__ finfo,
@@ -1423,7 +1423,8 @@ __global__ void __launch_bounds__(256) k_paint_tables(uint32_t P, const uint32_t
     const int lane = threadIdx.x & 63;
     const float4 src = make_float4(dec[c & 0xFFu], dec[(c >> 8) & 0xFFu], dec[(c >> 16) & 0xFFu], unorm_dec((int)(c >> 24)));
     const float a = src.w, oma = 1.f - a;
-    uint32_t e[4];
+    uint32_t e[4];  // e[ch]: channel ch's codes for k = 4l .. 4l + 3, one byte each
+    e[0] = e[1] = e[2] = e[3] = 0u;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int k = 4 * lane + j;
@@ -1436,15 +1437,21 @@ __global__ void __launch_bounds__(256) k_paint_tables(uint32_t P, const uint32_t
         const uint32_t c0 = SRGB ? (uint32_t)srgb_code(step, lut, v0, t) : (uint32_t)unorm_code(v0);
         const uint32_t c1 = SRGB ? (uint32_t)srgb_code(step, lut, v1, t) : (uint32_t)unorm_code(v1);
         const uint32_t c2 = SRGB ? (uint32_t)srgb_code(step, lut, v2, t) : (uint32_t)unorm_code(v2);
-        e[j] = c0 | (c1 << 8) | (c2 << 16) | ((uint32_t)unorm_code(va) << 24);
+        e[0] |= c0 << (8 * j);
+        e[1] |= c1 << (8 * j);
+        e[2] |= c2 << (8 * j);
+        e[3] |= (uint32_t)unorm_code(va) << (8 * j);
     }
-    *(uint4 *)(ptab + (size_t)path * 256 + 4 * lane) = make_uint4(e[0], e[1], e[2], e[3]);
+    uint32_t *t = ptab + (size_t)path * 256;
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) t[ch * 64 + lane] = e[ch];
 }
 
+// A code's blend through a planar table: channel ch's byte indexes bytes ch * 256 .. +255.
 __device__ __forceinline__ uint32_t table_code(const uint32_t *T, uint32_t st) {
     const uint8_t *b = (const uint8_t *)T;
-    return (uint32_t)b[(st & 0xFFu) * 4] | ((uint32_t)b[((st >> 8) & 0xFFu) * 4 + 1] << 8) |
-           ((uint32_t)b[((st >> 16) & 0xFFu) * 4 + 2] << 16) | ((uint32_t)b[(st >> 24) * 4 + 3] << 24);
+    return (uint32_t)b[st & 0xFFu] | ((uint32_t)b[256 + ((st >> 8) & 0xFFu)] << 8) |
+           ((uint32_t)b[512 + ((st >> 16) & 0xFFu)] << 16) | ((uint32_t)b[768 + (st >> 24)] << 24);
 }
 
 // One sample's framebuffer code blended with paint src (decoded colour, alpha a) and
@@ -1519,6 +1526,7 @@ __global__ void __launch_bounds__(256) k_composite_ms(CovArgs A) {
     __shared__ uint32_t l_pid[4][kCovList];
     __shared__ uint32_t slot[4][G][64];
     __shared__ uint32_t tbl[4][G][256];
+    __shared__ uint2 cmask[4][G][64];   // per column: both rows' sample bits (N = 32), or packed
     for (int i = threadIdx.x; i < 256; i += 256) {
         if (SRGB) s_step[i] = A.step_c[i];
         s_dec[i] = A.dec_c[i];
@@ -1608,7 +1616,9 @@ __global__ void __launch_bounds__(256) k_composite_ms(CovArgs A) {
                     cbh[g] = SPP == 32 ? hi : 0u;
                     if (GRAD) ce[g] = (uint32_t)px0 | ((uint32_t)(pw ? pw : 2) << 16);
                 }
+                cmask[w][g][lane] = make_uint2(cb[g], cbh[g]);
             }
+            wave_sync_lds();
             // blend, bucket by bucket in path order, one wave-wide pass per state register
             // holding a covered column: four LDS reads of the path's blend table for a
             // solid paint, the blend arithmetic on the pixel's own paint for a gradient.
@@ -1619,19 +1629,13 @@ __global__ void __launch_bounds__(256) k_composite_ms(CovArgs A) {
                 if (!regs) continue;
                 const float4 src = l_src[w][g0 + g];
                 const bool gpaint = GRAD && src.w < 0.f;
-                const uint32_t cbg = cb[g], cbhg = cbh[g], ceg = GRAD ? ce[g] : 0u;
-                // this lane's sample bit under register p's covering item
-                auto covers = [&](int p) -> bool {
-                    uint32_t bits;
-                    if (SPP == 32) {
-                        const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)cbg, p);
-                        const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)cbhg, p);
-                        bits = dy ? b1 : b0;
-                    } else {
-                        bits = (uint32_t)__shfl((int)cbg, p * CPR + q, 64) >> (dy * SPP);
-                    }
-                    return (bits >> smp) & 1u;
-                };
+                const uint32_t ceg = GRAD ? ce[g] : 0u;
+                // this lane's sample bit under register p's covering item: its column's
+                // masks from LDS (N = 32: the row's word; else both rows packed)
+                const uint32_t *cmw = (const uint32_t *)&cmask[w][g][0];
+                const int moff = SPP == 32 ? 2 * q + dy : 2 * q;
+                const int mshift = SPP == 32 ? smp : dy * SPP + smp;
+                auto covers = [&](int p) -> bool { return (cmw[2 * CPR * p + moff] >> mshift) & 1u; };
                 if (!gpaint) {
                     S.apply4(regs, [&](uint32_t st, int p) -> uint32_t {
                         const uint32_t nst = table_code(tbl[w][g], st);
