@@ -119,6 +119,7 @@ def cpu_baseline(scene, W, H, spp, ms, warmup, reps):
         if k >= warmup:
             times.append(r.seconds)
         frags = r.counts()["curve_fragment"]
+        print("cpu baseline: render %d of %d, %.2f s" % (k + 1, warmup + reps, r.seconds), file=sys.stderr, flush=True)
         del r
     t = statistics.median(times)
     return {"value": round(frags / t / 1e6, 3), "unit": "Mfragments/s", "cores": threads,
@@ -299,8 +300,11 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--ms", action="store_true", help="multisample (per-sample) compositing")
-    ap.add_argument("--cpu-warmup", type=int, default=2, help="untimed CPU-baseline renders")
-    ap.add_argument("--cpu-reps", type=int, default=10, help="timed CPU-baseline renders (median)")
+    ap.add_argument("--cpu-warmup", type=int, default=None,
+                    help="untimed CPU-baseline renders (default 2; 0 with --ms)")
+    ap.add_argument("--cpu-reps", type=int, default=None,
+                    help="timed CPU-baseline renders (median; default 10; 1 with --ms, whose CPU render "
+                         "blends every sample and takes about a minute)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
@@ -412,7 +416,9 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, W, H, args.spp, args.ms, args.cpu_warmup, args.cpu_reps)
+        warm = args.cpu_warmup if args.cpu_warmup is not None else (0 if args.ms else 2)
+        reps = args.cpu_reps if args.cpu_reps is not None else (1 if args.ms else 10)
+        cpu = cpu_baseline(scene, W, H, args.spp, args.ms, warm, reps)
 
     out = {
         "metric": "Mfragments/s + Mpixels/s at 2048^2 32spp (GB/s vs HBM peak); bwd/fwd time",
