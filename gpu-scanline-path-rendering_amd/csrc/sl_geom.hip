@@ -760,34 +760,71 @@ __device__ __forceinline__ void walk_piece(const PieceRef &r, const float *__res
 
 // Short and medium pieces (nx + ny + 1 < 128): one thread solves both sides and replays
 // the slot walk as it goes, each slot refilled by the next cut of its side -- the same
-// sequence walk_piece reads from the stored chains, without storing them.
-__device__ __forceinline__ void walk_piece_solving(PieceSetup &p, int2 *ts) {
-    int2 *out = ts + p.base;
-    const uint32_t nx = p.nx, ny = p.ny;
+// sequence walk_piece reads from the stored chains, without storing them.  The output is
+// staged: every lane walks kStageSteps entries of its piece into an LDS column, then the
+// wave writes the chunk out 16 lanes to a piece, so each piece's run of up to 128 bytes
+// goes out in one store instead of one 8-byte store a step (lines left partly written by
+// 64 pieces at once were evicted and written back several times: 1.16 GB written a frame
+// for 0.42 GB of ts on the 2048x2048 20k-path scene, 0.58 GB now).  A flag merge into an
+// entry already written out (equal t's across a chunk boundary, rare) waits for the
+// chunk's stores and then updates that entry in memory.
+constexpr int kStageSteps = 16;
+
+__device__ __forceinline__ void walk_pieces_staged(PieceSetup &p, bool live, int2 *ts, uint32_t *buf) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t nx = live ? p.nx : 0u, ny = live ? p.ny : 0u;
+    const uint32_t n_loop = live ? nx + ny + 1 : 0u;
+    const uint32_t base = live ? p.base : 0u, curve = live ? p.c : 0u;
     const float two = 2.f, two1 = bf(fb(2.f) | 1u);
-    float xs, ys;
-    uint32_t rx, ry;
-    if (nx == 0) { xs = two; ys = p.t0m; rx = 0; ry = ny; }
-    else if (ny == 0) { xs = p.t0m; ys = two1; rx = nx; ry = 0; }
-    else { xs = p.sx.next(); rx = nx - 1; ys = p.t0m; ry = ny; }
+    float xs = two, ys = two1;
+    uint32_t rx = 0, ry = 0;
+    if (live) {
+        if (nx == 0) { xs = two; ys = p.t0m; rx = 0; ry = ny; }
+        else if (ny == 0) { xs = p.t0m; ys = two1; rx = nx; ry = 0; }
+        else { xs = p.sx.next(); rx = nx - 1; ys = p.t0m; ry = ny; }
+    }
+    uint32_t nmax = n_loop;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor((int)nmax, d, 64));
     uint32_t last = fb(-1.f);
-    const uint32_t n_loop = nx + ny + 1;
-    for (uint32_t it = 0; it < n_loop; ++it) {
-        float v;
-        if (xs <= ys) {
-            v = xs;
-            if (rx > 0) { xs = p.sx.next(); rx--; } else { xs = two; }
-        } else {
-            v = ys;
-            if (ry > 0) { ys = p.sy.next(); ry--; } else { ys = two1; }
+    for (uint32_t it0 = 0; it0 < nmax; it0 += kStageSteps) {
+#pragma unroll
+        for (int j = 0; j < kStageSteps; ++j) {
+            const uint32_t it = it0 + j;
+            if (it < n_loop) {
+                float v;
+                if (xs <= ys) {
+                    v = xs;
+                    if (rx > 0) { xs = p.sx.next(); rx--; } else { xs = two; }
+                } else {
+                    v = ys;
+                    if (ry > 0) { ys = p.sy.next(); ry--; } else { ys = two1; }
+                }
+                uint32_t o = fb(v);
+                if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
+                    o |= last;
+                    if (j > 0) {
+                        buf[(j - 1) * 64 + lane] = o;
+                    } else {
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the chunk's stores first
+                        ts[base + it - 1].y = (int)o;
+                    }
+                }
+                buf[j * 64 + lane] = o;
+                last = o;
+            }
         }
-        uint32_t o = fb(v);
-        if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
-            o |= last;
-            out[it - 1].y = (int)o;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll 4
+        for (int k = 0; k < 16; ++k) {
+            const int L = 4 * k + (lane >> 4), j = lane & 15;
+            const uint32_t nL = (uint32_t)__shfl((int)n_loop, L, 64);
+            const uint32_t bL = (uint32_t)__shfl((int)base, L, 64), cL = (uint32_t)__shfl((int)curve, L, 64);
+            if (it0 + j < nL) ts[bL + it0 + j] = make_int2((int)cL, (int)buf[j * 64 + L]);
         }
-        out[it] = make_int2((int)p.c, (int)o);
-        last = o;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -805,11 +842,16 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
         solve_chain(sc, tv, ccuts, cncut, coff, pcnt, clist[i], W, H, chain_t);
         return;
     }
+    __shared__ uint32_t stage[4][kStageSteps * 64];
     const uint32_t g = (blockIdx.x - chain_blocks) * blockDim.x + threadIdx.x;
-    if (g >= class_total(cnt, kShortClass0, kListClasses)) return;
-    const uint32_t d = plist[cnt[kLongPieceClass] + g];
-    PieceSetup p = piece_setup(sc, tv, ccuts, cncut, coff, pcnt, d >> 3, (int)(d & 7u), W, H);
-    walk_piece_solving(p, ts);
+    const bool live = g < class_total(cnt, kShortClass0, kListClasses);
+    if (!__ballot(live)) return;  // whole wave past the list
+    PieceSetup p;
+    if (live) {
+        const uint32_t d = plist[cnt[kLongPieceClass] + g];
+        p = piece_setup(sc, tv, ccuts, cncut, coff, pcnt, d >> 3, (int)(d & 7u), W, H);
+    }
+    walk_pieces_staged(p, live, ts, stage[threadIdx.x >> 6]);
 }
 
 // Long pieces: one wavefront per piece merges in parallel.  The x chain X and the y
