@@ -333,6 +333,24 @@ __global__ void k_d_composite(DiffDev D, const float *__restrict__ cov, const fl
     img[i] = make_float4(I0, I1, I2, I3);
 }
 
+// Gradient sums are accumulated in 64-bit fixed point, 2^-32 units: each contribution is
+// rounded once (to nearest) and added with an integer atomic.  Integer addition is
+// associative, so the sums do not depend on the order the atomics land in and every run
+// gives the same bits; a float atomicAdd's result depends on that order.  Contributions
+// are clamped to |v| <= 2^30 and the sums stay far inside the int64 range for any
+// gradient image of moderate size.
+constexpr double kFixScale = 4294967296.0;  // 2^32
+
+__device__ __forceinline__ void fix_add(unsigned long long *acc, float v) {
+    const double c = fmin(fmax((double)v, -1073741824.0), 1073741824.0);
+    atomicAdd(acc, (unsigned long long)__double2ll_rn(c * kFixScale));
+}
+
+__global__ void k_d_unfix(const long long *__restrict__ acc, float *__restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)((double)acc[i] * (1.0 / kFixScale));
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -342,7 +360,8 @@ __device__ __forceinline__ float wave_sum(float v) {
 // Adjoint of the compositing chain, one thread per pixel.
 __global__ void __launch_bounds__(256) k_d_back_pixels(DiffDev D, const float *__restrict__ cov,
                                                        const float4 *__restrict__ col, const float4 *__restrict__ pre,
-                                                       const float4 *__restrict__ gimg, float *gcov, float *gcol) {
+                                                       const float4 *__restrict__ gimg, float *gcov,
+                                                       unsigned long long *gcol) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t NP = (int64_t)D.W * D.H;
     const bool in = i < NP;
@@ -360,10 +379,10 @@ __global__ void __launch_bounds__(256) k_d_back_pixels(DiffDev D, const float *_
         }
         const float s0 = wave_sum(g.x * a), s1 = wave_sum(g.y * a), s2 = wave_sum(g.z * a), s3 = wave_sum(da * cv);
         if ((threadIdx.x & 63) == 0 && (s0 != 0.f || s1 != 0.f || s2 != 0.f || s3 != 0.f)) {
-            atomicAdd(&gcol[4 * p + 0], s0);
-            atomicAdd(&gcol[4 * p + 1], s1);
-            atomicAdd(&gcol[4 * p + 2], s2);
-            atomicAdd(&gcol[4 * p + 3], s3);
+            fix_add(&gcol[4 * p + 0], s0);
+            fix_add(&gcol[4 * p + 1], s1);
+            fix_add(&gcol[4 * p + 2], s2);
+            fix_add(&gcol[4 * p + 3], s3);
         }
         const float oma = 1.f - a;
         g.x *= oma;
@@ -376,7 +395,7 @@ __global__ void __launch_bounds__(256) k_d_back_pixels(DiffDev D, const float *_
 // Boundary terms at the crossings, one thread per sorted crossing.
 __global__ void k_d_back_cross(DiffDev D, uint32_t NX, const uint32_t *__restrict__ order, const Cross *__restrict__ xs,
                                const int8_t *__restrict__ df, const float2 *__restrict__ pts,
-                               const float *__restrict__ gcov, float *gpts) {
+                               const float *__restrict__ gcov, unsigned long long *gpts) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= NX) return;
     const int f = df[k];
@@ -400,8 +419,8 @@ __global__ void k_d_back_cross(DiffDev D, uint32_t NX, const uint32_t *__restric
     const int nv = g.type & 7;
     const uint32_t v0 = D.cvpos[c];
     for (int v = 0; v < nv; ++v) {
-        atomicAdd(&gpts[2 * (size_t)(v0 + v)], gx * b[v]);
-        atomicAdd(&gpts[2 * (size_t)(v0 + v) + 1], -gx * ratio * b[v]);
+        fix_add(&gpts[2 * (size_t)(v0 + v)], gx * b[v]);
+        fix_add(&gpts[2 * (size_t)(v0 + v) + 1], -gx * ratio * b[v]);
     }
 }
 
@@ -425,6 +444,7 @@ struct sl_diff {
     uint32_t P = 0, C = 0, V = 0;
     Buf cvpos, cpath, ctype, frule;
     Buf cnt, off, xs, key, bcount, boff, bcur, order, sx, sd, df, cov, pre, gcov, scan_tmp;
+    Buf gfix;  // fixed-point gradient sums: 2V point components, then 4P colour components
     uint32_t NX = 0;
     bool have_pre = false;
     const float *last_points = nullptr, *last_colors = nullptr;
@@ -545,7 +565,8 @@ int sl_diff_destroy(sl_diff *d) {
     (void)hipSetDevice(d->device);
     (void)hipDeviceSynchronize();
     Buf *bs[] = {&d->cvpos, &d->cpath, &d->ctype, &d->frule, &d->cnt, &d->off, &d->xs, &d->key, &d->bcount,
-                 &d->boff, &d->bcur, &d->order, &d->sx, &d->sd, &d->df, &d->cov, &d->pre, &d->gcov, &d->scan_tmp};
+                 &d->boff, &d->bcur, &d->order, &d->sx, &d->sd, &d->df, &d->cov, &d->pre, &d->gcov, &d->scan_tmp,
+                 &d->gfix};
     for (Buf *b : bs)
         if (b->p) (void)hipFree(b->p);
     if (d->done_ev) (void)hipEventDestroy(d->done_ev);
@@ -624,17 +645,20 @@ int sl_diff_backward(sl_diff *d, const float *grad_image, float *grad_points, fl
     const DiffDev D = dev_view(d);
     const size_t NP = (size_t)d->W * d->H;
     int rc;
-    if ((rc = dens(d, d->gcov, (size_t)d->P * NP * 4))) return rc;
-    hipLaunchKernelGGL(k_d_fill, dim3(256), dim3(256), 0, s, grad_points, (size_t)d->V * 2);
-    hipLaunchKernelGGL(k_d_fill, dim3(4), dim3(256), 0, s, grad_colors, (size_t)d->P * 4);
+    const size_t npt = (size_t)d->V * 2, ncol = (size_t)d->P * 4;
+    if ((rc = dens(d, d->gcov, (size_t)d->P * NP * 4)) || (rc = dens(d, d->gfix, (npt + ncol) * 8))) return rc;
+    unsigned long long *fix_pts = BP<unsigned long long>(d->gfix), *fix_col = fix_pts + npt;
+    hipLaunchKernelGGL(k_d_fill, dim3(256), dim3(256), 0, s, BP<float>(d->gfix), (npt + ncol) * 2);
     hipLaunchKernelGGL(k_d_back_pixels, dim3(nblk((int64_t)NP, 256)), dim3(256), 0, s, D, (const float *)BP<float>(d->cov),
                        (const float4 *)d->last_colors, (const float4 *)BP<float4>(d->pre), (const float4 *)grad_image,
-                       BP<float>(d->gcov), grad_colors);
+                       BP<float>(d->gcov), fix_col);
     if (d->NX)
         hipLaunchKernelGGL(k_d_back_cross, dim3(nblk(d->NX, 256)), dim3(256), 0, s, D, d->NX,
                            (const uint32_t *)BP<uint32_t>(d->order), (const Cross *)BP<Cross>(d->xs),
                            (const int8_t *)BP<int8_t>(d->df), (const float2 *)d->last_points,
-                           (const float *)BP<float>(d->gcov), grad_points);
+                           (const float *)BP<float>(d->gcov), fix_pts);
+    hipLaunchKernelGGL(k_d_unfix, dim3(256), dim3(256), 0, s, (const long long *)fix_pts, grad_points, npt);
+    hipLaunchKernelGGL(k_d_unfix, dim3(16), dim3(256), 0, s, (const long long *)fix_col, grad_colors, ncol);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return dfail(d, SL_E_HIP, "backward launch failed: %s", hipGetErrorString(e));
     return diff_issued(d, s);
