@@ -113,6 +113,16 @@ struct sl_handle {
     void *noalloc_scratch = nullptr;
     int64_t NB = 0;
     uint32_t *nz_vals_sorted = nullptr;
+    // Status of the last single-scene frame, copied back without waiting at the end of the
+    // frame into pinned memory (words 0..7 of the frame's small block: big-bucket count,
+    // error flags, then the merged-fragment and item totals) and read when the frame is
+    // settled: the element counts of an eager frame, and every frame's error flags.
+    uint32_t *fstat = nullptr;
+    bool stat_pending = false;   // a queued frame writes fstat
+    bool stat_counts = false;    // ...including the element counts (an eager frame)
+    bool timing_pending = false; // ...and its stage events are to be read
+    int frame_rc = SL_OK;        // error of the last settled frame
+    std::string frame_err;
 };
 
 namespace {
@@ -279,12 +289,37 @@ int check_launch(sl_handle *h, const char *what) {
     return SL_OK;
 }
 
-uint32_t read_u32(sl_handle *h, const uint32_t *dptr, hipStream_t s, int &rc) {
-    uint32_t v = 0;
-    hipError_t e = hipMemcpyAsync(&v, dptr, 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) rc = fail(h, SL_E_HIP, "count read-back failed: %s", hipGetErrorString(e));
-    return v;
+// The status a settled frame left in fstat: its error flags and, for an eager frame, the
+// element counts and stage times.
+void collect_status(sl_handle *h) {
+    if (!h->stat_pending) return;
+    h->stat_pending = false;
+    const uint32_t *st = h->fstat;
+    h->frame_rc = SL_OK;
+    h->frame_err.clear();
+    if (st[1] & 1u) {
+        h->frame_rc = SL_E_HIP;
+        h->frame_err = "internal error: fragment row outside its path's bucket range";
+    } else if (st[1] & 2u) {
+        h->frame_rc = SL_E_UNSUPPORTED;
+        h->frame_err = "a (path, row) bucket holds more than 65536 fragments";
+    }
+    if (h->stat_counts) {
+        unsigned long long M = 0, MS = 0;
+        memcpy(&M, st + 4, 8);
+        memcpy(&MS, st + 6, 8);
+        h->counts.merged_fragment = (int64_t)M;
+        h->counts.span = (int64_t)(MS - M);
+        h->stat_counts = false;
+    }
+    if (h->timing_pending) {
+        for (int i = 0; i < kStages; ++i) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]);
+            h->stage_ms[i] = ms;
+        }
+        h->timing_pending = false;
+    }
 }
 
 // One frame.  `replay` re-issues the work of the previous eager render of the same scene
@@ -429,14 +464,20 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
     sl::scan_exclusive_u32(f.cfrag, f.coff, C, f.scan_tmp, s);
     sl::scan_exclusive_u32(f.nchain, f.choff, C, f.scan_tmp, s);
     if ((rc = check_launch(h, "monotone"))) return rc;
-    uint32_t F = 0, NCH = 0;
+    // the bucket ranges need only the paths' row ranges, final after the monotone pass, so
+    // an eager frame reads its three sizes (fragments, chains, buckets) in one round trip
+    sl::launch_path_buckets(sc, f, s);
+    sl::scan_exclusive_u32(f.pnb, f.pbase, P, f.scan_tmp, s);
+    uint32_t F = 0, NCH = 0, NB = 0;
     if (replay) {
         F = h->last_F;
         NCH = h->last_NCH;
+        NB = h->last_NB;
     } else {
         if (!carved) h->sized = false;
         HIPCHK(h, hipMemcpyAsync(&F, f.coff + C, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(h, hipMemcpyAsync(&NCH, f.choff + C, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(h, hipMemcpyAsync(&NB, f.pbase + P, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(h, hipStreamSynchronize(s));
         h->counts.path = P; h->counts.curve = C; h->counts.vertex = V;
         h->counts.curve_fragment = F;
@@ -470,10 +511,6 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
     sl::launch_intersect(sc, f, s);
     if ((rc = check_launch(h, "intersect"))) return rc;
     mark(3);
-    sl::launch_path_buckets(sc, f, s);
-    sl::scan_exclusive_u32(f.pnb, f.pbase, P, f.scan_tmp, s);
-    const uint32_t NB = replay ? h->last_NB : read_u32(h, f.pbase + P, s, rc);
-    if (rc) return rc;
     f.NB = NB;
     if (!replay) h->NB = NB;
     const size_t R = (size_t)f.R;
@@ -538,12 +575,33 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
     sl::launch_composite(sc, tb, f, s);
     if ((rc = check_launch(h, "composite"))) return rc;
     mark(8);
-    if (replay) return SL_OK;
+    // a single-scene frame (batches and parity dumps keep the synchronous read below) hands
+    // its status back without waiting: a replay its error flags, an eager frame also the
+    // element counts; the host reads them when it settles the frame
+    const bool deferred = !h->nscenes && !h->debug;
+    if (replay) {
+        if (deferred) {
+            // words 4..7 (the counts) keep what an eager frame still pending wrote: a replay's
+            // counts are its eager frame's
+            HIPCHK(h, hipMemcpyAsync(h->fstat, A.b_small.p, 8, hipMemcpyDeviceToHost, s));
+            h->stat_counts = h->stat_pending && h->stat_counts;
+            h->stat_pending = true;
+        }
+        return SL_OK;
+    }
 
     // element counts (M, S) and the error flag
     unsigned long long *red = (unsigned long long *)(P_<uint32_t>(A.b_small) + 4);
     sl::launch_fill_u32((uint32_t *)red, 0u, 4, s);
     sl::launch_count_reduce(f.bmcnt, f.bicnt, NB, red, s);
+    if (deferred) {
+        HIPCHK(h, hipMemcpyAsync(h->fstat, A.b_small.p, 32, hipMemcpyDeviceToHost, s));
+        h->stat_pending = true;
+        h->stat_counts = true;
+        h->timing_pending = h->timing;
+        if (!carved) { h->last_F = F; h->last_NCH = NCH; h->last_NB = NB; h->sized = true; }
+        return SL_OK;
+    }
     uint32_t small[8];
     HIPCHK(h, hipMemcpyAsync(small, A.b_small.p, 32, hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
@@ -635,6 +693,7 @@ static int settle(sl_handle *h) {
         HIPCHK(h, hipEventSynchronize(h->done_ev));
         h->pending = false;
     }
+    collect_status(h);
     return SL_OK;
 }
 
@@ -671,9 +730,19 @@ int sl_create(int device, sl_handle **out) {
         return SL_E_HIP;
     }
     h->events_ready = true;
+    if (hipHostMalloc((void **)&h->fstat, 64, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        for (int k = 0; k <= kStages; ++k) (void)hipEventDestroy(h->ev[k]);
+        (void)hipEventDestroy(h->done_ev);
+        (void)hipEventDestroy(h->switch_ev);
+        delete h;
+        return SL_E_OOM;
+    }
+    memset(h->fstat, 0, 64);
     // set-up of the allocation-free frame: this GPU's address of the static scratch
     if (sl::noalloc_scratch(device, &h->noalloc_scratch) != hipSuccess) {
         (void)hipGetLastError();
+        (void)hipHostFree(h->fstat);
         for (int k = 0; k <= kStages; ++k) (void)hipEventDestroy(h->ev[k]);
         (void)hipEventDestroy(h->done_ev);
         (void)hipEventDestroy(h->switch_ev);
@@ -698,6 +767,7 @@ int sl_destroy(sl_handle *h) {
                       &h->A.b_rrec, &h->A.b_btile, &h->A.b_rtile, &h->A.b_rpath, &h->A.b_ptab, &h->A.b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry,
                       &h->gramp, &h->gtable};
     for (DevBuf *b : bufs) release(*b);
+    if (h->fstat) (void)hipHostFree(h->fstat);
     if (h->stage) (void)hipHostFree(h->stage);
     if (h->events_ready) {
         for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(h->ev[i]);
@@ -1080,6 +1150,15 @@ int sl_rasterize_noalloc(sl_handle *h, void *dev_rgba, int pitch_bytes, void *st
         (void)hipStreamSynchronize(s);
         return rc;
     }
+    // the frame's status comes back when it is settled, as for sl_rasterize
+    if (hipEventRecord(h->done_ev, s) == hipSuccess) {
+        h->pending = true;
+        h->pending_stream = s;
+    } else {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(s);
+        h->pending = false;
+    }
     return SL_OK;
 }
 
@@ -1096,6 +1175,7 @@ int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags) {
     int rc = check_image(h);
     if (rc) return rc;
     if ((rc = settle(h))) return rc;
+    if (h->frame_rc) return fail(h, h->frame_rc, "%s", h->frame_err.c_str());
     const size_t row = (size_t)h->img_W * 4;
     HIPCHK(h, hipMemcpy(host_rgba, h->A.b_image.p, row * h->img_H, hipMemcpyDeviceToHost));
     if (flags & 1) flip_rows_alpha(host_rgba, h->img_W, h->img_H);
@@ -1104,7 +1184,9 @@ int sl_read_output(sl_handle *h, uint8_t *host_rgba, int flags) {
 
 int sl_element_number(sl_handle *h, sl_counts *out) {
     if (!h || !out) return SL_E_ARG;
+    if (int rc = settle(h)) return rc;
     *out = h->counts;
+    if (h->frame_rc) return fail(h, h->frame_rc, "%s", h->frame_err.c_str());
     return SL_OK;
 }
 
@@ -1116,6 +1198,7 @@ int sl_enable_step_timing(sl_handle *h, int enable) {
 
 int sl_step_timing(sl_handle *h, double *ms, int capacity, int *n_stages) {
     if (!h || !n_stages) return SL_E_ARG;
+    if (int rc = settle(h)) return rc;
     *n_stages = kStages;
     if (ms)
         for (int i = 0; i < kStages && i < capacity; ++i) ms[i] = h->stage_ms[i];

@@ -157,11 +157,15 @@ int sl_clear(sl_handle *h);
  * scene into the handle's device RGBA8 image (row 0 = y in [0,1), the reference's
  * GL framebuffer order).  `stream` is a hipStream_t or NULL.
  * The first frame after any change (scene, transform, output size, sample count, output
- * mode, colour encoding, debug dumps) reads its counts back and returns when the frame is
- * done.  Later frames of the unchanged scene reuse those counts and sizes: every kernel
- * runs again, queued on `stream` with no host synchronisation, and the call returns at
- * once (step timing and debug dumps keep every frame synchronous).  sl_read_output waits
- * for such a frame; users of sl_output_device_ptr order their work after it on `stream`.
+ * mode, colour encoding, debug dumps) reads its three sizes (fragments, chains, buckets)
+ * back in one round trip early in the frame, sizes its buffers and returns with the rest
+ * queued.  Later frames of the unchanged scene reuse those sizes: every kernel runs
+ * again, queued on `stream` with no host synchronisation, and the call returns at once.
+ * Every frame hands its element counts and error flags back without waiting; they are
+ * read when the frame is settled: sl_read_output and sl_element_number wait for the frame
+ * and return its error (SL_E_UNSUPPORTED for a (path, row) bucket of more than 65536
+ * fragments), users of sl_output_device_ptr order their work after it on `stream`.
+ * Parity dumps keep every frame synchronous.
  * A queued frame shares the handle's buffers with the next one: sl_add_vg, sl_debug_dump
  * and a full frame on the same stream wait for it on the host.  A frame issued on another
  * stream comes after everything queued on the previous frame's stream up to the call (that

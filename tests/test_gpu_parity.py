@@ -562,3 +562,42 @@ def test_viewport_present():
         r.viewport(0, 0, 20, 10)
     finally:
         r.close()
+
+
+def test_transform_changes_queue_without_status_reads():
+    """A frame under a changed transform reads its three sizes back in one round trip and
+    returns without waiting for the frame; its element counts and error flags come back
+    when the frame is settled.  Frames under three transforms queued back to back, then
+    replays of the last: counts and image are the oracle's for the last transform."""
+    import time
+    scene = scanline.generate(scanline.scene.KIND_MIXED, 640, 480, 400, 0x7A5F)
+    arrs = scene.arrays() if hasattr(scene, "arrays") else scene
+    r = scanline.VGRasterizer(0)
+    try:
+        r.setSamples(16)
+        r.enableSRGBCorrection(True)
+        r.setMultisampleOutput(False)
+        r.resizeOutputBuffer(640, 480)
+        r.addVg(arrs)
+        mats = []
+        for k in range(3):
+            m = np.eye(4, dtype=np.float32)
+            m[0, 0], m[1, 1] = 0.8 + 0.1 * k, 1.05 - 0.05 * k
+            m[3, 0], m[3, 1] = 5.25 * k + 0.5, -3.75 * k + 0.25
+            mats.append(m)
+        for m in mats:
+            print("queued transform frame", float(m[0, 0]), flush=True)
+            r.setInputTransform(m)
+            t0 = time.perf_counter()
+            r.rasterize()
+            print("  call returned after %.2f ms" % (1e3 * (time.perf_counter() - t0)), flush=True)
+        r.rasterize()
+        r.rasterize()
+        ref = pyoracle.run(arrs, 640, 480, 16, srgb=True, ms=False, transform=mats[-1], keep=False)
+        got = r.elementNumber()
+        want = ref.counts()
+        for k in ("curve_fragment", "merged_fragment", "span"):
+            assert got[k] == want[k], (k, got, want)
+        np.testing.assert_array_equal(r.output(), ref.get("image").reshape(480, 640, 4))
+    finally:
+        r.close()
