@@ -212,7 +212,8 @@ struct ResolveArgs {
     uint32_t *bicnt;
     uint32_t *bmcnt;
     uint32_t *bext;
-    uint64_t *btile;         // [NB] 64-pixel tiles of the row holding items (W <= 4096)
+    uint64_t *btile;         // [NB] 64-pixel tiles holding items, bit k = the k-th tile from the
+                             // one holding the extent's start (bit 63: that tile or any beyond)
     uint32_t *big_list;
     uint32_t *big_count;
     uint32_t *err;
@@ -248,12 +249,15 @@ struct OpOr { __device__ int operator()(int a, int b) const { return a | b; } };
 
 // 64-pixel tiles spanned by [lo, hi) as bits of a 64-bit mask (all tiles when the canvas
 // is wider than 64 tiles), OR-reduced over the wave into lane 63.
-__device__ __forceinline__ uint64_t tile_bits(int lo, int hi, int W) {
+__device__ __forceinline__ uint64_t tile_bits(int lo, int hi, int tb) {
     if (lo >= hi) return 0ull;
-    if (W > 4096) return ~0ull;
-    const int t0 = lo >> 6, t1 = (hi - 1) >> 6;
+    const int t0 = tmin((lo >> 6) - tb, 63), t1 = tmin(((hi - 1) >> 6) - tb, 63);
     return (~0ull << t0) & (~0ull >> (63 - t1));
 }
+
+// The bit a tile starting at tx0 tests in the bitmap of a bucket whose extent starts at
+// exlo (tiles at or beyond the 63rd from the extent's first all test bit 63).
+__device__ __forceinline__ int tile_bit(int tx0, int exlo) { return tmax(tmin((tx0 >> 6) - (exlo >> 6), 63), 0); }
 __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(dpp_scan((int)(uint32_t)v, 0, OpOr()), 63);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(dpp_scan((int)(uint32_t)(v >> 32), 0, OpOr()), 63);
@@ -408,7 +412,7 @@ __device__ __forceinline__ void resolve_bucket(const ResolveArgs &A, int64_t b, 
         const int hi = merged ? x + 2 : (span ? x : INT_MIN);
         xmin = tmin(xmin, lane63(dpp_scan(lo, INT_MAX, OpMin())));
         xmax = tmax(xmax, lane63(dpp_scan(hi, INT_MIN, OpMax())));
-        tmask |= wave_or64(tile_bits(lo, hi, W));
+        tmask |= wave_or64(tile_bits(lo, hi, xmin >> 6));
         nmerged += lane63(scan_add(merged ? 1 : 0));
     }
     if (lane == 0) {
@@ -512,7 +516,7 @@ __device__ __forceinline__ void resolve_small(const ResolveArgs &A, int64_t b, u
     const int xmax = lane63(dpp_scan(hi, INT_MIN, OpMax()));
     const int nmerged = lane63(scan_add(merged ? 1 : 0));
     const int icnt = lane63(ic);
-    const uint64_t tmask = wave_or64(tile_bits(lo, hi, W));
+    const uint64_t tmask = wave_or64(tile_bits(lo, hi, xmin >> 6));
     if (lane == 0) {
         A.bicnt[b] = (uint32_t)icnt;
         A.bmcnt[b] = (uint32_t)nmerged;
@@ -604,7 +608,7 @@ __device__ __forceinline__ void resolve_pack(const ResolveArgs &A, int64_t b0, i
     const int xmin = dpp_scan_seg<S>(lo, INT_MAX, OpMin());
     const int xmax = dpp_scan_seg<S>(hi, INT_MIN, OpMax());
     const int nmerged = dpp_scan_seg<S>(merged ? 1 : 0, 0, OpAdd());
-    const uint64_t tb = tile_bits(lo, hi, W);
+    const uint64_t tb = tile_bits(lo, hi, xmin >> 6);  // a lane's prefix minimum is its segment's first item
     const uint32_t tlo = (uint32_t)dpp_scan_seg<S>((int)(uint32_t)tb, 0, OpOr());
     const uint32_t thi = (uint32_t)dpp_scan_seg<S>((int)(uint32_t)(tb >> 32), 0, OpOr());
     if (k >= 0 && i == S - 1) {
@@ -728,9 +732,12 @@ __global__ void __launch_bounds__(64) k_resolve_big(ResolveArgs A) {
     }
 }
 
-// Buckets of more than 1024 fragments (rare): bitonic-sort (cx << 16 | index) keys in a
-// global scratch region, then walk the sorted order in chunks of 64 reading the records
-// from global memory.
+// Buckets of more than 1024 fragments (rare): bitonic-sort 64-bit (cell x, index) keys,
+// their high words in the bucket's stretch of the global scratch and their low words in
+// its stretch of the item array (2n words each; the padded power of two is at most 2n),
+// copy the sorted indices into the scratch (the items are written over next), then walk
+// the sorted order in chunks of 64 reading the records from global memory.  Any bucket
+// size works.
 template <int SPP>
 __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
     constexpr int NW = Words<SPP>::n;
@@ -741,29 +748,36 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
         const uint32_t base = A.boff[b];
         const int n = (int)(A.boff[b + 1] - base);
         if (n <= 64 * kBigStage) continue;
-        if (n > 65536) {
-            if (lane == 0) { atomicOr(A.err, 2u); A.bicnt[b] = 0; A.bmcnt[b] = 0; A.bext[b] = 0xFFFFu; A.btile[b] = 0ull; }
-            continue;
-        }
         int P2 = 1;
         while (P2 < n) P2 <<= 1;
-        uint32_t *keys = A.scratch + 2 * (size_t)base;
-        for (int i = lane; i < P2; i += 64)
-            keys[i] = (i < n) ? (((A.rec_meta[base + i] & 0xFFFFu) << 16) | (uint32_t)i) : 0xFFFFFFFFu;
+        uint32_t *khi = A.scratch + 2 * (size_t)base;
+        uint32_t *klo = A.items + 2 * (size_t)base;
+        for (int i = lane; i < P2; i += 64) {
+            khi[i] = (i < n) ? (A.rec_meta[base + i] & 0xFFFFu) : 0xFFFFFFFFu;
+            klo[i] = (i < n) ? (uint32_t)i : 0xFFFFFFFFu;
+        }
         __syncthreads();
         for (int k = 2; k <= P2; k <<= 1) {
             for (int j = k >> 1; j > 0; j >>= 1) {
                 for (int i = lane; i < P2; i += 64) {
                     const int l = i ^ j;
                     if (l > i) {
-                        const uint32_t a = keys[i], c = keys[l];
+                        const uint32_t ah = khi[i], al = klo[i], ch = khi[l], cl = klo[l];
+                        const bool gt = ah > ch || (ah == ch && al > cl);
                         const bool up = (i & k) == 0;
-                        if ((a > c) == up) { keys[i] = c; keys[l] = a; }
+                        if (gt == up) {
+                            khi[i] = ch; klo[i] = cl;
+                            khi[l] = ah; klo[l] = al;
+                        }
                     }
                 }
                 __syncthreads();
             }
         }
+        for (int i = lane; i < n; i += 64) khi[i] = klo[i];
+        __syncthreads();
+        const uint32_t *order = khi;  // sorted fragment indices; a fragment's cell x is in its record
+        auto cell_x = [&](int q) { return (int)(A.rec_meta[base + order[q]] & 0xFFFFu); };
         const int rule = A.frule[A.binfo[b].x];
         uint32_t wcarry = A.bcarry[b];
         int W = A.W;
@@ -778,11 +792,10 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int p = c0 + lane;
             const bool v = p < n;
-            const uint32_t kk = v ? keys[p] : 0xFFFFFFFFu;
-            const int cx = (int)(kk >> 16);
-            const uint32_t meta = v ? A.rec_meta[base + (kk & 0xFFFFu)] : 0u;
+            const uint32_t meta = v ? A.rec_meta[base + order[p]] : 0u;
+            const int cx = (int)(meta & 0xFFFFu);
             const int swn = v ? (int)(int8_t)(meta >> 16) : 0;
-            const int prev_cx = (v && p > 0) ? (int)(keys[p - 1] >> 16) : -1;
+            const int prev_cx = (v && p > 0) ? cell_x(p - 1) : -1;
             const bool head = v && (p == 0 || cx != prev_cx);
             const int incl = scan_add(swn);
             const int8_t w_at = (int8_t)(int)(wcarry + (uint32_t)(incl - swn));
@@ -794,9 +807,9 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
                 Stencil<NW> st;
                 st.init(w_at);
                 for (int q = p; q < n; ++q) {
-                    const uint32_t kq = keys[q];
-                    if ((int)(kq >> 16) != cx) break;
-                    const uint32_t j = kq & 0xFFFFu;
+                    const uint32_t j = order[q];
+                    const uint32_t mq = A.rec_meta[base + j];
+                    if ((int)(mq & 0xFFFFu) != cx) break;
                     uint32_t m2[NW];
                     load_mask<NW>(A.rec_mask, (size_t)base + j, m2);
                     st.add(m2, ((A.rec_meta[base + j] >> 24) & 1u) != 0u);
@@ -822,7 +835,7 @@ __global__ void __launch_bounds__(64) k_resolve_huge(ResolveArgs A) {
             const int hi = merged ? x + 2 : (span ? x : INT_MIN);
             xmin = tmin(xmin, lane63(dpp_scan(lo, INT_MAX, OpMin())));
             xmax = tmax(xmax, lane63(dpp_scan(hi, INT_MIN, OpMax())));
-            tmask |= wave_or64(tile_bits(lo, hi, W));
+            tmask |= wave_or64(tile_bits(lo, hi, xmin >> 6));
             nmerged += lane63(scan_add(merged ? 1 : 0));
         }
         if (lane == 0) {
@@ -1281,7 +1294,6 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_comp
     };
 
     const uint32_t q0 = A.rowbeg[row0 + r], q1 = A.rowend[row0 + r];
-    const int tbit = (tx0 >> 6) & 63;
     int nlist = 0;
     for (uint32_t qc = q0; qc < q1; qc += 64) {
         const bool inq = qc + lane < q1;
@@ -1290,7 +1302,7 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 4) k_comp
         const int exlo = (int)(rec.x & 0xFFFFu), exhi = (int)(rec.x >> 16);
         // a word without alpha paints nothing, unless it names a gradient that paints
         const bool paints = (rec.y & 0xFF000000u) || (GRAD && rec.y != 0u && grad_paints(A.G, rec.y));
-        const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tbit) & 1ull) && paints;
+        const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tile_bit(tx0, exlo)) & 1ull) && paints;
         const unsigned long long om = __ballot(ov);
         if (!om) continue;
         const int nov = (int)__popcll(om);
@@ -1657,7 +1669,6 @@ __global__ void __launch_bounds__(256) k_composite_ms(CovArgs A) {
     };
 
     const uint32_t q0 = A.rowbeg[r], q1 = A.rowend[r];
-    const int tbit = (tx0 >> 6) & 63;
     int nlist = 0;
     for (uint32_t qc = q0; qc < q1; qc += 64) {
         const bool inq = qc + lane < q1;
@@ -1665,7 +1676,7 @@ __global__ void __launch_bounds__(256) k_composite_ms(CovArgs A) {
         const uint64_t rt = inq ? A.rtile[qc + lane] : 0ull;
         const int exlo = (int)(rec.x & 0xFFFFu), exhi = (int)(rec.x >> 16);
         const bool paints = (rec.y & 0xFF000000u) || (GRAD && rec.y != 0u && grad_paints(A.G, rec.y));
-        const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tbit) & 1ull) && paints;
+        const bool ov = inq && exlo < tx1 && exhi > tx0 && ((rt >> tile_bit(tx0, exlo)) & 1ull) && paints;
         const unsigned long long om = __ballot(ov);
         if (!om) continue;
         const int nov = (int)__popcll(om);

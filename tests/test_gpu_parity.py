@@ -601,3 +601,20 @@ def test_transform_changes_queue_without_status_reads():
         np.testing.assert_array_equal(r.output(), ref.get("image").reshape(480, 640, 4))
     finally:
         r.close()
+
+
+def test_bucket_beyond_65536_fragments(rasterizer):
+    """One path whose single cell row holds about 160k fragments (80 near-horizontal edges
+    across a 4096-px row): the largest-bucket resolve sorts 64-bit (cell x, index) keys, so
+    any bucket size renders, and every stage matches the oracle."""
+    vg = VGContainer()
+    vg.addPath(1, rgba(30, 120, 200, 200))
+    pts = []
+    for k in range(40):
+        y = 10.1 + 0.04 * k
+        pts.append((0.5, y))
+        pts.append((4095.5, y + 0.02))
+    vg.addPolygon(pts)
+    vg.addPath(0, rgba(200, 60, 20))
+    vg.addPolygon([(100.0, 2.0), (300.0, 2.0), (300.0, 14.0), (100.0, 14.0)])
+    compare(rasterizer, vg, 4096, 16, 8, label="huge bucket")
