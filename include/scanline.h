@@ -163,8 +163,8 @@ int sl_clear(sl_handle *h);
  * again, queued on `stream` with no host synchronisation, and the call returns at once.
  * Every frame hands its element counts and error flags back without waiting; they are
  * read when the frame is settled: sl_read_output and sl_element_number wait for the frame
- * and return its error (SL_E_UNSUPPORTED for a (path, row) bucket of more than 65536
- * fragments), users of sl_output_device_ptr order their work after it on `stream`.
+ * and return its error (SL_E_HIP for an internal inconsistency flagged on the device),
+ * users of sl_output_device_ptr order their work after it on `stream`.
  * Parity dumps keep every frame synchronous.
  * A queued frame shares the handle's buffers with the next one: sl_add_vg, sl_debug_dump
  * and a full frame on the same stream wait for it on the host.  A frame issued on another
