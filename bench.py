@@ -348,6 +348,126 @@ def bench_diff(args):
     print(json.dumps(out))
 
 
+def bench_animation(args):
+    """Animation mode (the reference's -animation): two chord diagrams of 150 nodes and
+    about 1,800 links an interval on a 1920x1080 window, 16 spp coverage output.  One step
+    = one frame: the scene at the frame's time generated on the GPU from keyframes already
+    on the device, then rendered in full (the geometry moves, so every frame sizes its
+    buffers from the device's counts).  Frame times sweep the three keyframe intervals."""
+    import numpy as np
+    import torch
+
+    import scanline
+    from scanline import animation as an
+
+    W, H = an.CANVAS
+    spp = 16
+    ds = an.refugee_diagrams(n_nodes=150, n_years=4, links_per_year=1500, seed=0)
+    r = scanline.VGRasterizer(0)
+    r.setSamples(spp)
+    r.setMultisampleOutput(False)
+    r.enableSRGBCorrection(True)
+    r.resizeOutputBuffer(W, H)
+    anim = an.ChordAnimation(r, ds)
+    stream = torch.cuda.current_stream().cuda_stream
+    span = 3 * an.KEY_MS
+    times = [an.KEY_MS + (k * span) // max(1, args.steps) for k in range(args.steps)]
+    for k in range(args.warmup):
+        anim.frame(times[k % len(times)], stream)
+        r.rasterize(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in times:
+        anim.frame(t, stream)
+        r.rasterize(stream)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) / len(times) * 1e3
+
+    # the generation kernel alone, HIP events on the stream it runs on
+    reps = 200
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for k in range(reps):
+        anim.frame(times[k % len(times)], stream)
+    ev1.record()
+    torch.cuda.synchronize()
+    gen_us = ev0.elapsed_time(ev1) / reps * 1e3
+    P_mid = anim.paths_at(times[len(times) // 2])
+    # bytes one generation writes (per path: 24 vertices of 8 + 4 bytes, 6 curves of
+    # 4+1+4+4+1+16 bytes, fill rule and word) and reads (its keyframe rows, ~64 bytes)
+    gen_bytes = P_mid * (24 * 12 + 6 * 30 + 5 + 64)
+
+    # stage times of the renderer over the same frames
+    r.enableStepTiming(True)
+    stage_sum, frame_ms = {}, []
+    for t in times:
+        anim.frame(t, stream)
+        r.rasterize(stream)
+        st = r.stepTiming()
+        frame_ms.append(sum(st.values()))
+        for k, v in st.items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    r.enableStepTiming(False)
+    stages = {k: v / len(times) for k, v in stage_sum.items()}
+    cnt = r.elementNumber()
+    dom = max((k for k in stages if stage_bytes(k, cnt, spp, W, H) > 0), key=lambda k: stages[k])
+    dom_bytes = stage_bytes(dom, cnt, spp, W, H)
+    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import statistics
+
+        import pyoracle
+
+        L = pyoracle.lib(omp=True)
+        t_mid = times[len(times) // 2]
+        scene = pyoracle.chord_scene(pyoracle.chord_frame(ds, t_mid))
+        secs = []
+        t_all = time.perf_counter()
+        for k in range(4):
+            res = pyoracle.run(scene, W, H, spp, srgb=True, keep=False, omp=True)
+            secs.append(res.seconds)
+            print("cpu baseline: render %d of 4, %.2f s" % (k + 1, res.seconds), file=sys.stderr, flush=True)
+            del res
+        med = statistics.median(secs[1:])
+        cpu = {"value": round(1.0 / med, 3), "unit": "frames/s", "cores": L.oracle_threads(), "kind": "port",
+               "sample": "the frame at t = %d ms (%d paths, %dx%d, %d spp, coverage): median of 3 renders "
+                         "after 1 warm-up, %.3f s each; %.0f s in all" % (t_mid, scene.n_paths, W, H, spp, med,
+                                                                           time.perf_counter() - t_all)}
+
+    out = {
+        "metric": "animation frames/s (GPU-generated chord scene + full render, 1920x1080 16spp)",
+        "value": round(1e3 / ms_step, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": len(times),
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded flow records: 150 nodes, 4 keyframes, 1500 links a year)",
+        "config": {"workload": "chord animation, 2 diagrams x 150 nodes, ~1800 links an interval, "
+                               "1920x1080, 16 spp coverage", "width": W, "height": H, "spp": spp,
+                   "paths_mid": P_mid},
+        "generate_us": round(gen_us, 2),
+        "generate_gbs": round(gen_bytes / (gen_us * 1e-6) / 1e9, 2),
+        "frame_event_ms_median": round(float(np.median(frame_ms)), 4),
+        "counts": cnt,
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL.get((dom, False), dom), "stage": dom,
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "alg_bytes": int(dom_bytes), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(out["value"] / cpu["value"], 2)
+    print(json.dumps(out))
+    r.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -365,7 +485,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration)")
-    ap.add_argument("--mode", choices=["plain", "batch", "diff"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "batch", "diff", "animation"], default="plain")
     ap.add_argument("--config", type=int, choices=[2, 3], default=2,
                     help="plain mode workload: 2 = 2048^2 mixed paths (headline), 3 = text-like 4096^2")
     ap.add_argument("--batch-scenes", type=int, default=4096)
@@ -381,6 +501,9 @@ def main():
         return
     if args.mode == "diff":
         bench_diff(args)
+        return
+    if args.mode == "animation":
+        bench_animation(args)
         return
 
     import numpy as np

@@ -123,6 +123,16 @@ struct sl_handle {
     bool timing_pending = false; // ...and its stage events are to be read
     int frame_rc = SL_OK;        // error of the last settled frame
     std::string frame_err;
+    // keyframes of the scene generated on the device (sl_chord_load), one block on the
+    // device; per diagram the byte offsets of its arrays in that block
+    struct Chord {
+        int n_keys = 0, n_nodes = 0;
+        std::vector<uint32_t> link_offset;
+        float cx = 0.f, cy = 0.f;
+        size_t o_t0 = 0, o_t1 = 0, o_size = 0, o_color = 0, o_node = 0, o_from = 0, o_to = 0;
+    };
+    std::vector<Chord> chord;
+    DevBuf chord_keys;
 };
 
 namespace {
@@ -765,7 +775,7 @@ int sl_destroy(sl_handle *h) {
                       &h->A.b_big_list, &h->A.b_small, &h->A.b_nz_flag, &h->A.b_nz_off, &h->A.b_nzk0, &h->A.b_nzk1, &h->A.b_nzv0,
                       &h->A.b_nzv1, &h->A.b_rs_hist, &h->A.b_rs_hoff, &h->A.b_rowbeg, &h->A.b_rowend, &h->A.b_image, &h->A.b_scan_tmp,
                       &h->A.b_rrec, &h->A.b_btile, &h->A.b_rtile, &h->A.b_rpath, &h->A.b_ptab, &h->A.b_lcount, &h->pcanvas, &h->stile, &h->simg, &h->sfirst, &h->scarry,
-                      &h->gramp, &h->gtable};
+                      &h->gramp, &h->gtable, &h->chord_keys};
     for (DevBuf *b : bufs) release(*b);
     if (h->fstat) (void)hipHostFree(h->fstat);
     if (h->stage) (void)hipHostFree(h->stage);
@@ -1089,6 +1099,161 @@ int sl_clear(sl_handle *h) {
     return SL_OK;
 }
 
+int sl_chord_load(sl_handle *h, const sl_chord *d, int n) {
+    if (!h) return SL_E_ARG;
+    if (!d || n < 1 || n > sl::kMaxChordDiagrams)
+        return fail(h, SL_E_ARG, "need 1 to %d chord diagrams (got %d)", sl::kMaxChordDiagrams, d ? n : 0);
+    // validation first: nothing changes on failure
+    uint64_t paths = 0;
+    for (int i = 0; i < n; ++i) {
+        const sl_chord &c = d[i];
+        if (c.n_keys < 2 || c.n_nodes < 0) return fail(h, SL_E_ARG, "diagram %d: n_keys %d < 2 or n_nodes %d < 0", i, c.n_keys, c.n_nodes);
+        if (!c.link_offset) return fail(h, SL_E_ARG, "diagram %d: link_offset is NULL", i);
+        if (c.link_offset[0] != 0) return fail(h, SL_E_ARG, "diagram %d: link_offset[0] must be 0", i);
+        uint32_t most = 0;
+        for (int k = 1; k < c.n_keys; ++k) {
+            if (c.link_offset[k] < c.link_offset[k - 1])
+                return fail(h, SL_E_ARG, "diagram %d: link_offset decreases at keyframe %d", i, k);
+            most = std::max(most, c.link_offset[k] - c.link_offset[k - 1]);
+        }
+        const uint32_t L = c.link_offset[c.n_keys - 1];
+        if (c.n_nodes && (!c.node_t0 || !c.node_t1 || !c.node_size || !c.node_color))
+            return fail(h, SL_E_ARG, "diagram %d: node array pointer is NULL", i);
+        if (L && (!c.link_node || !c.link_from || !c.link_to))
+            return fail(h, SL_E_ARG, "diagram %d: link array pointer is NULL", i);
+        for (uint32_t l = 0; l < 2 * (uint64_t)L; ++l)
+            if (c.link_node[l] >= (uint32_t)c.n_nodes)
+                return fail(h, SL_E_ARG, "diagram %d: link %u names node %u >= n_nodes %d", i, l / 2, c.link_node[l], c.n_nodes);
+        paths += (uint64_t)c.n_nodes + most;
+    }
+    if (paths > (1u << 24)) return fail(h, SL_E_UNSUPPORTED, "chord scenes of more than 2^24 paths (%llu)", (unsigned long long)paths);
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    int rc = settle(h);
+    if (rc) return rc;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    std::vector<sl_handle::Chord> ch(n);
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const sl_chord &c = d[i];
+        sl_handle::Chord &q = ch[i];
+        const size_t KN = (size_t)c.n_keys * c.n_nodes, L = c.link_offset[c.n_keys - 1];
+        q.n_keys = c.n_keys;
+        q.n_nodes = c.n_nodes;
+        q.link_offset.assign(c.link_offset, c.link_offset + c.n_keys);
+        q.cx = c.center_x;
+        q.cy = c.center_y;
+        q.o_t0 = total; total += al(4 * KN);
+        q.o_t1 = total; total += al(4 * KN);
+        q.o_size = total; total += al(4 * KN);
+        q.o_color = total; total += al(4 * (size_t)c.n_nodes);
+        q.o_node = total; total += al(8 * L);
+        q.o_from = total; total += al(16 * L);
+        q.o_to = total; total += al(16 * L);
+    }
+    std::vector<uint8_t> blk(total);
+    for (int i = 0; i < n; ++i) {
+        const sl_chord &c = d[i];
+        const sl_handle::Chord &q = ch[i];
+        const size_t KN = (size_t)c.n_keys * c.n_nodes, L = c.link_offset[c.n_keys - 1];
+        if (KN) {
+            std::memcpy(&blk[q.o_t0], c.node_t0, 4 * KN);
+            std::memcpy(&blk[q.o_t1], c.node_t1, 4 * KN);
+            std::memcpy(&blk[q.o_size], c.node_size, 4 * KN);
+        }
+        if (c.n_nodes) std::memcpy(&blk[q.o_color], c.node_color, 4 * (size_t)c.n_nodes);
+        if (L) {
+            std::memcpy(&blk[q.o_node], c.link_node, 8 * L);
+            std::memcpy(&blk[q.o_from], c.link_from, 16 * L);
+            std::memcpy(&blk[q.o_to], c.link_to, 16 * L);
+        }
+    }
+    h->has_scene = false;
+    h->P = h->C = h->V = 0;
+    h->sized = false;
+    h->chord.clear();
+    if ((rc = upload(h, h->chord_keys, blk.data(), total))) return rc;
+    h->chord = std::move(ch);
+    return SL_OK;
+}
+
+int sl_chord_frame(sl_handle *h, int time_ms, void *stream) {
+    if (!h) return SL_E_ARG;
+    if (h->chord.empty()) return fail(h, SL_E_STATE, "no chord keyframes loaded (sl_chord_load)");
+    if (time_ms < 0) return fail(h, SL_E_ARG, "time %d ms is negative", time_ms);
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
+    const int n = (int)h->chord.size();
+    sl::ChordInterval iv[sl::kMaxChordDiagrams];
+    uint32_t P = 0;
+    const uint8_t *base = (const uint8_t *)h->chord_keys.p;
+    for (int i = 0; i < n; ++i) {
+        const sl_handle::Chord &q = h->chord[i];
+        // keyframe pair of the frame (chord_animation, animation.cu:2457-2459); past the
+        // last keyframe the diagram rests on it
+        int k = time_ms / sl::kChordKeyMs + 1;
+        float t = sl::chord_ease(time_ms);
+        if (k > q.n_keys - 1) {
+            k = q.n_keys - 1;
+            t = 1.f;
+        }
+        const uint32_t l0 = q.link_offset[k - 1], nl = q.link_offset[k] - l0;
+        const size_t a = (size_t)(k - 1) * q.n_nodes, b = (size_t)k * q.n_nodes;
+        sl::ChordInterval &v = iv[i];
+        v.n_nodes = q.n_nodes;
+        v.n_links = (int)nl;
+        v.t = t;
+        v.center = make_float2(q.cx, q.cy);
+        v.node_t0_a = (const float *)(base + q.o_t0) + a;
+        v.node_t0_b = (const float *)(base + q.o_t0) + b;
+        v.node_t1_a = (const float *)(base + q.o_t1) + a;
+        v.node_t1_b = (const float *)(base + q.o_t1) + b;
+        v.node_size_a = (const float *)(base + q.o_size) + a;
+        v.node_size_b = (const float *)(base + q.o_size) + b;
+        v.node_color = (const uint32_t *)(base + q.o_color);
+        v.link_node = (const uint2 *)(base + q.o_node) + l0;
+        v.link_from = (const float4 *)(base + q.o_from) + l0;
+        v.link_to = (const float4 *)(base + q.o_to) + l0;
+        P += (uint32_t)q.n_nodes + nl;
+    }
+    const uint32_t C = 6 * P, V = 24 * P;
+    const hipStream_t s = (hipStream_t)stream;
+    struct Part { DevBuf *b; size_t bytes; };
+    const Part parts[] = {{&h->vtx, 8 * (size_t)V}, {&h->vpath, 4 * (size_t)V}, {&h->cvpos, 4 * (size_t)C},
+                          {&h->ctype, C},           {&h->cpath, 4 * (size_t)C}, {&h->cw, 4 * (size_t)C},
+                          {&h->crev, C},            {&h->cinfo, 16 * (size_t)C}, {&h->frule, P},
+                          {&h->finfo, 4 * (size_t)P}};
+    bool grow = false;
+    for (const Part &q : parts) grow |= q.b->cap < std::max<size_t>(q.bytes, 16);
+    if (grow) {
+        // a queued frame may still read the arrays about to be replaced
+        int rc = settle(h);
+        if (rc) return rc;
+        for (const Part &q : parts)
+            if ((rc = ensure(h, *q.b, q.bytes))) return rc;
+    }
+    // the scene arrays are rewritten after every frame issued before this call
+    if (h->pending && s != h->pending_stream) {
+        HIPCHK(h, hipEventRecord(h->switch_ev, h->pending_stream));
+        HIPCHK(h, hipStreamWaitEvent(s, h->switch_ev, 0));
+    }
+    sl::ChordScene out{(float2 *)h->vtx.p,   (uint32_t *)h->vpath.p, (uint32_t *)h->cvpos.p, (uint32_t *)h->cpath.p,
+                       (uint32_t *)h->finfo.p, (uint8_t *)h->ctype.p, (uint8_t *)h->frule.p, (uint8_t *)h->crev.p,
+                       (float *)h->cw.p,     (uint4 *)h->cinfo.p};
+    sl::launch_chord_frame(iv, n, P, out, s);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventRecord(h->done_ev, s));
+    h->pending = true;
+    h->pending_stream = s;
+    h->P = P;
+    h->C = C;
+    h->V = V;
+    h->has_scene = true;
+    h->sized = false;
+    h->has_grad = false;
+    h->gslots = 0;
+    h->gramp_rows = 0;
+    return ensure_tables(h);
+}
+
 int sl_rasterize(sl_handle *h, void *stream) {
     if (!h) return SL_E_ARG;
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, SL_E_HIP, "hipSetDevice failed");
@@ -1307,6 +1472,18 @@ int sl_debug_dump(sl_handle *h, const char *name, void *host_dst, size_t capacit
         n = (size_t)h->img_W * h->img_H * 4;
         *nbytes = n;
         if (host_dst && capacity >= n) HIPCHK(h, hipMemcpy(host_dst, h->A.b_image.p, n, hipMemcpyDeviceToHost));
+        return SL_OK;
+    }
+    // the scene arrays as the renderer reads them (an upload's or a generated frame's)
+    struct SceneDump { const char *name; const DevBuf *b; size_t bytes; };
+    const SceneDump sd[] = {{"scene_vertex", &h->vtx, 8 * (size_t)h->V},  {"scene_vertex_path", &h->vpath, 4 * (size_t)h->V},
+                            {"scene_curve", &h->cinfo, 16 * (size_t)h->C}, {"scene_fill_rule", &h->frule, h->P},
+                            {"scene_fill", &h->finfo, 4 * (size_t)h->P}};
+    for (const SceneDump &q : sd) {
+        if (k != q.name) continue;
+        n = h->has_scene ? q.bytes : 0;
+        *nbytes = n;
+        if (host_dst && capacity >= n && n) HIPCHK(h, hipMemcpy(host_dst, q.b->p, n, hipMemcpyDeviceToHost));
         return SL_OK;
     }
     if (k == "pixel_table") {

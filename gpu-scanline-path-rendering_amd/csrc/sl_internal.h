@@ -179,6 +179,30 @@ void launch_encode_sweep(const Tables &t, int channel, uint32_t lo, uint32_t hi,
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launch_valu_probe(int blocks, int iters, float *out, hipStream_t s);
 void launch_fill_rows_u8(uint8_t *p, uint8_t v, size_t row_bytes, size_t pitch, size_t rows, hipStream_t s);
+
+// GPU-side scene generation (sl_anim.hip): one diagram's interval between keyframes
+// k - 1 (_a) and k (_b), device pointers, and the eased weight t of the frame
+constexpr int kMaxChordDiagrams = 8;
+constexpr int kChordKeyMs = 6000;  // time from one keyframe to the next (animation.cu:1708)
+struct ChordInterval {
+    int n_nodes, n_links;
+    float t;
+    float2 center;
+    const float *node_t0_a, *node_t0_b, *node_t1_a, *node_t1_b, *node_size_a, *node_size_b;  // [n_nodes]
+    const uint32_t *node_color;                                                                // [n_nodes]
+    const uint2 *link_node;                  // [n_links] the two nodes
+    const float4 *link_from, *link_to;       // [n_links] {pos0, pos1, size0, size1} at k - 1, k
+};
+// the scene arrays a frame writes (6 curves and 24 vertex slots per path)
+struct ChordScene {
+    float2 *vtx;
+    uint32_t *vpath, *cvpos, *cpath, *finfo;
+    uint8_t *ctype, *frule, *crev;
+    float *cw;
+    uint4 *cinfo;
+};
+float chord_ease(int time_ms);
+void launch_chord_frame(const ChordInterval *d, int n_diagrams, uint32_t P, const ChordScene &out, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 
 // static scratch of the allocation-free frame (sl_noalloc.hip): its size, and its device

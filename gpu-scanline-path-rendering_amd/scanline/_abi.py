@@ -18,7 +18,8 @@ FILL_NONZERO, FILL_EVEN_ODD = 0, 1
 EXPORTS = (
     "sl_create", "sl_destroy", "sl_set_samples", "sl_set_multisample_output",
     "sl_enable_srgb_correction", "sl_set_input_transform", "sl_resize_output_buffer",
-    "sl_viewport", "sl_resize_viewport", "sl_present", "sl_add_vg", "sl_clear", "sl_rasterize", "sl_rasterize_noalloc", "sl_output_device_ptr", "sl_read_output",
+    "sl_viewport", "sl_resize_viewport", "sl_present", "sl_add_vg", "sl_clear", "sl_chord_load",
+    "sl_chord_frame", "sl_rasterize", "sl_rasterize_noalloc", "sl_output_device_ptr", "sl_read_output",
     "sl_element_number", "sl_enable_step_timing", "sl_step_timing", "sl_step_name",
     "sl_pixel_count", "sl_last_error", "sl_debug_enable", "sl_debug_dump",
     "sl_debug_requantize", "sl_debug_requantize_sweep", "sl_debug_valu_probe", "sl_batch_create", "sl_batch_destroy",
@@ -45,6 +46,23 @@ class SlScene(ctypes.Structure):
         ("gradient_ramp_size", ctypes.c_uint32),
         ("gradient_table", ctypes.c_void_p),
         ("gradient_table_size", ctypes.c_uint32),
+    ]
+
+
+class SlChord(ctypes.Structure):
+    _fields_ = [
+        ("n_keys", ctypes.c_int32),
+        ("n_nodes", ctypes.c_int32),
+        ("node_t0", ctypes.c_void_p),
+        ("node_t1", ctypes.c_void_p),
+        ("node_size", ctypes.c_void_p),
+        ("node_color", ctypes.c_void_p),
+        ("link_offset", ctypes.c_void_p),
+        ("link_node", ctypes.c_void_p),
+        ("link_from", ctypes.c_void_p),
+        ("link_to", ctypes.c_void_p),
+        ("center_x", ctypes.c_float),
+        ("center_y", ctypes.c_float),
     ]
 
 
@@ -91,6 +109,8 @@ def lib():
         "sl_present": (i, [vp, vp, i, i, i, vp]),
         "sl_add_vg": (i, [vp, ctypes.POINTER(SlScene)]),
         "sl_clear": (i, [vp]),
+        "sl_chord_load": (i, [vp, ctypes.POINTER(SlChord), i]),
+        "sl_chord_frame": (i, [vp, i, vp]),
         "sl_rasterize": (i, [vp, vp]),
         "sl_rasterize_noalloc": (i, [vp, vp, i, vp]),
         "sl_output_device_ptr": (i, [vp, ctypes.POINTER(vp), ctypes.POINTER(i)]),

@@ -18,6 +18,8 @@ DUMP_DTYPES = {
     "bucket_info": (np.int32, 2), "bucket_items": (np.uint32, 1), "bucket_ext": (np.uint32, 1),
     "bucket_off": (np.uint32, 1), "row_keys": (np.uint32, 1), "row_list": (np.uint32, 1),
     "row_beg": (np.uint32, 1), "row_end": (np.uint32, 1),
+    "scene_vertex": (np.float32, 2), "scene_vertex_path": (np.uint32, 1), "scene_curve": (np.uint32, 4),
+    "scene_fill_rule": (np.uint8, 1), "scene_fill": (np.uint32, 1),
 }
 
 

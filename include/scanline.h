@@ -151,6 +151,50 @@ int sl_add_vg(sl_handle *h, const sl_scene *scene);
 /* clear (ras_base.cpp:572-573): drop the uploaded scene. */
 int sl_clear(sl_handle *h);
 
+/* ---- scenes generated on the device ------------------------------------------------
+ * The reference's animation mode writes each frame's scene on the GPU straight into the
+ * rasterizer's input packs (vg_animation -> chord_animation, animation.cu:2435-2786,
+ * called from the frame loop at ras_cut_mask_comb_scanline.cu:903-921).  Here a caller
+ * loads the keyframes once and asks for the scene at a time; the next sl_rasterize renders
+ * it.  Keyframes of one chord diagram, as RefugeeAnimation::genFrame lays them out
+ * (animation.cu:1200-1330):
+ *   node_t0, node_t1, node_size  [n_keys * n_nodes]  angular extent (radians, direction
+ *                                 (sin t, cos t) from the centre) and size at each keyframe
+ *   node_color                   [n_nodes] RGBA8 fill word of each node (a link takes its
+ *                                 larger end's colour with the alpha's top bit cleared)
+ *   link_offset                  [n_keys] the links drawn while moving from keyframe k - 1
+ *                                 to k are [link_offset[k - 1], link_offset[k]); entry 0 is 0
+ *   link_node                    [2 * n_links] the two nodes of each link
+ *   link_from, link_to           [4 * n_links] {pos0, pos1, size0, size1} of each link's
+ *                                 two ends inside their nodes, at keyframe k - 1 and k
+ *   center_x, center_y           the diagram's centre in scene coordinates
+ * Every node becomes a ring sector between radii 320 and 350, every link a ribbon inside
+ * radius 320; each path is 6 curves in 24 vertex slots, nodes before links, diagram after
+ * diagram.  The country and year text of the reference's demo is not generated. */
+typedef struct sl_chord {
+    int32_t         n_keys;           /* >= 2 */
+    int32_t         n_nodes;
+    const float    *node_t0, *node_t1, *node_size;
+    const uint32_t *node_color;
+    const uint32_t *link_offset;
+    const uint32_t *link_node;
+    const float    *link_from, *link_to;
+    float           center_x, center_y;
+} sl_chord;
+
+/* load_refugee_animation's result (animation.cu:1480-1706) handed to the device: copies
+ * up to 8 diagrams' keyframes to device memory and drops the handle's scene.  The caller's
+ * arrays are not retained. */
+int sl_chord_load(sl_handle *h, const sl_chord *diagrams, int n_diagrams);
+
+/* chord_animation at time `time_ms` >= 0 (animation.cu:2435-2760): keyframes k - 1 and
+ * k = time_ms / 6000 + 1 eased by GET_ANIMATION_T (animation.cu:1710-1718); after the last
+ * keyframe the diagram holds it.  The scene is written on `stream` into the handle's scene
+ * arrays, already oriented as sl_add_vg leaves an upload, and replaces the scene; the next
+ * sl_rasterize is a full frame.  It is queued after every frame issued before it; the call
+ * waits on the host only when the scene outgrows its arrays. */
+int sl_chord_frame(sl_handle *h, int time_ms, void *stream);
+
 /* ---- rendering ---------------------------------------------------------------------- */
 
 /* rasterize() (ras_base.cpp:1131-1195 -> rasterizeImpl<2>, .cu:879-1453): renders the

@@ -60,6 +60,15 @@ def lib(omp=False):
     L.oracle_curve_eval.restype = ctypes.c_float
     L.oracle_curve_eval.argtypes = [i, ctypes.c_float, vp, ctypes.c_float]
     L.oracle_color_tables.argtypes = [i, vp, vp]
+    if hasattr(L, "oracle_chord_diagram"):
+        f = ctypes.c_float
+        L.oracle_chord_sincos.argtypes = [f, ctypes.POINTER(f), ctypes.POINTER(f)]
+        L.oracle_chord_ease.restype = f
+        L.oracle_chord_ease.argtypes = [i]
+        L.oracle_chord_paths.restype = ctypes.c_int64
+        L.oracle_chord_paths.argtypes = [i, i, vp, i]
+        L.oracle_chord_diagram.restype = ctypes.c_int64
+        L.oracle_chord_diagram.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp, f, f, i, u32, vp, vp, vp, vp, vp]
     if hasattr(L, "oracle_diff_run"):
         L.oracle_diff_run.argtypes = [i, i, i, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                       ctypes.POINTER(ctypes.c_int64)]
@@ -146,3 +155,53 @@ def diff_run(scene, width, height, points=None, colors=None, grad_image=None, sa
     if rc:
         raise ValueError("oracle_diff_run: unsupported scene (rational segments?)")
     return img, gp, gc, n.value
+
+
+def chord_sincos(x):
+    """(sin, cos) of a float angle as the chord generator evaluates them."""
+    s, c = ctypes.c_float(), ctypes.c_float()
+    lib().oracle_chord_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+def chord_ease(time_ms):
+    return lib().oracle_chord_ease(int(time_ms))
+
+
+def chord_frame(diagrams, time_ms):
+    """The scene of `diagrams` (scanline.animation.ChordKeyframes) at time_ms, as dicts of
+    the arrays the renderer dumps: vertex [V, 2] f32, vertex_path [V], curve [C, 4] u32
+    {vertex pos, path, type | reversed << 8, weight bits}, fill_rule [P] u8, fill [P] u32."""
+    L = lib()
+    counts = [L.oracle_chord_paths(d.n_keys, d.n_nodes, d.link_offset.ctypes.data, int(time_ms)) for d in diagrams]
+    P = int(sum(counts))
+    out = {"vertex": np.zeros((24 * P, 2), np.float32), "vertex_path": np.zeros(24 * P, np.uint32),
+           "curve": np.zeros((6 * P, 4), np.uint32), "fill_rule": np.zeros(P, np.uint8),
+           "fill": np.zeros(P, np.uint32)}
+    base = 0
+    for d, n in zip(diagrams, counts):
+        got = L.oracle_chord_diagram(d.n_keys, d.n_nodes, d.node_t0.ctypes.data, d.node_t1.ctypes.data,
+                                     d.node_size.ctypes.data, d.node_color.ctypes.data, d.link_offset.ctypes.data,
+                                     d.link_node.ctypes.data, d.link_from.ctypes.data, d.link_to.ctypes.data,
+                                     float(d.center[0]), float(d.center[1]), int(time_ms), base,
+                                     out["vertex"].ctypes.data, out["vertex_path"].ctypes.data,
+                                     out["curve"].ctypes.data, out["fill_rule"].ctypes.data, out["fill"].ctypes.data)
+        assert got == n
+        base += int(n)
+    return out
+
+
+def chord_scene(frame):
+    """A chord_frame result as scanline.scene.SceneArrays, as a host upload of the same
+    curves would give it: each curve back in the order it was generated in (the reversed
+    flag keeps its direction, which the winding depends on), so that the upload orients
+    it again and derives the same flag."""
+    from scanline.scene import SceneArrays
+    cur = frame["curve"]
+    vtx = frame["vertex"].copy()
+    for vp, _, info, _ in cur[(cur[:, 2] >> 8) & 1 == 1]:
+        nv = int(info & 7)
+        vtx[vp:vp + nv] = vtx[vp:vp + nv][::-1].copy()
+    return SceneArrays(vertex=vtx.reshape(-1), curve_vertex_pos=cur[:, 0].copy(),
+                       curve_type=(cur[:, 2] & 0xFF).astype(np.uint8), curve_path_id=cur[:, 1].copy(),
+                       curve_arc_w=np.ones(cur.shape[0], np.float32), fill_rule=frame["fill_rule"].copy(), fill_color=frame["fill"].copy())

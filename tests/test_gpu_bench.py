@@ -46,3 +46,11 @@ def test_diff_mode():
     d = run_bench("--mode", "diff", "--steps", "2", "--warmup", "1")
     assert COMMON <= set(d)
     assert d["forward_ms"] > 0 and d["backward_ms"] > 0
+
+
+def test_animation_mode():
+    d = run_bench("--mode", "animation", "--steps", "3", "--warmup", "1", "--no-cpu-baseline")
+    assert COMMON <= set(d)
+    assert d["unit"] == "frames/s" and d["value"] > 0 and d["generate_us"] > 0
+    assert {"bound", "achieved", "peak", "unit", "frac", "traffic"} <= set(d["roofline"])
+    assert d["counts"]["path"] > 0 and d["counts"]["curve"] == 6 * d["counts"]["path"]
