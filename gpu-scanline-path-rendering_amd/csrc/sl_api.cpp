@@ -447,7 +447,7 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
         (rc = ensure_in(h, A, A.b_pcnt, (size_t)C * 20)) || (rc = ensure_in(h, A, A.b_nchain, ((size_t)C + 1) * 4)) ||
         (rc = ensure_in(h, A, A.b_choff, ((size_t)C + 1) * 4)) ||
         (rc = ensure_in(h, A, A.b_pbase, ((size_t)P + 1) * 4)) || (rc = ensure_in(h, A, A.b_small, 64)) ||
-        (rc = ensure_in(h, A, A.b_lcount, 1024)) ||
+        (rc = ensure_in(h, A, A.b_lcount, sl::kListCountBytes)) ||
         (!carved && (rc = ensure_in(h, A, A.b_image, image_bytes))) ||
         (rc = ensure_in(h, A, A.b_scan_tmp, scan_need * 4)))
         return rc;

@@ -405,26 +405,25 @@ __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__r
 // loops of the same kind and about the same trip count and the long sequential walks
 // start early.  A count pass sizes the classes; the place pass puts every entry at its
 // class offset plus a rank taken per block in LDS and one reservation per class and block.
-constexpr int kLenBins = 13;
+constexpr int kLenBins = 25;
 constexpr int kChainClasses = 2 * kLenBins;                     // cubic bins, then closed-form bins
 constexpr int kLongPieceClass = kChainClasses;                  // one class
 constexpr int kShortClass0 = kChainClasses + 1;                 // cubic bins, then closed-form bins
 constexpr int kListClasses = kShortClass0 + 2 * kLenBins;
 constexpr uint32_t kLongPiece = 128;
 constexpr int kChainListThreads = 1024;
+static_assert(2 * kListClasses * sizeof(uint32_t) <= kListCountBytes, "class counts and cursors exceed their buffer");
 
-// length bins, longest first: >=128, 65-127, 33-64, 25-32, 17-24, 13-16, 9-12, 7-8, 5-6, 4, 3, 2, 1
+// Length bins, longest first.  Below 256 a bin's longest entry is at most 4/3 of its
+// shortest, so that the lanes of a wave run loops of nearly equal trip counts.  The bin
+// of L is the number of lower bounds above L.
 __device__ __forceinline__ int len_bin(uint32_t L) {
-    if (L >= 128) return 0;
-    if (L > 64) return 1;
-    if (L > 32) return 2;
-    if (L > 24) return 3;
-    if (L > 16) return 4;
-    if (L > 12) return 5;
-    if (L > 8) return 6;
-    if (L > 6) return 7;
-    if (L > 4) return 8;
-    return 13 - (int)L;
+    constexpr uint32_t lower[kLenBins] = {384, 256, 192, 160, 128, 112, 96, 80, 65, 57, 49, 41, 33,
+                                          29,  25,  21,  17,  13,  9,   7,  5,  4,  3,  2,  1};
+    int b = 0;
+#pragma unroll
+    for (int i = 0; i < kLenBins; ++i) b += lower[i] > L ? 1 : 0;
+    return b;
 }
 
 struct CurvePieces {

@@ -74,6 +74,9 @@ struct GradDev {
                              //   (vg_main.cpp:117-143)
 };
 
+// bytes of the intersection work lists' class counts and cursors (Frame::lcount)
+constexpr size_t kListCountBytes = 1024;
+
 struct Frame {
     int W, H;
     float m[16];             // rows of the input transform, row-major
@@ -90,7 +93,7 @@ struct Frame {
     uint32_t *choff;         // [C+1] exclusive scan of nchain
     uint32_t *chains;        // [NCH] solve chains of long pieces, grouped by class
     uint32_t *plist;         // [NCH/2] pieces: long ones, then short ones grouped by class
-    uint32_t *lcount;        // [2 * 53] class counts (26 chain, 1 long-piece, 26 short-piece
+    uint32_t *lcount;        // [2 * 101] class counts (50 chain, 1 long-piece, 50 short-piece
                              //   classes), then the place pass's cursors
     int64_t NCH;
     float *chain_t;          // [F] solved t | side per chain
