@@ -1523,12 +1523,19 @@ __device__ __forceinline__ uint32_t column_bits(const uint32_t *m, int dx, uint3
     return (m[0] >> (16 * dx)) & 0xFFFFu;
 }
 
+// Occupancy over group size: two buckets a group (half the LDS and prefetch registers of
+// four) with registers capped for 4 / 5 / 6 waves per SIMD at 32 / 16 / 8 samples runs the
+// headline scene's multisample composite in 13.9 / 7.4 / 4.6 ms against 16.1 / 8.1 / 5.7 ms
+// for four-bucket groups at 3 / 4 / 5 waves (at 32 samples the cap spills 32 bytes a lane,
+// still faster than one-bucket groups without spills, 14.5 ms).  The gradient variants,
+// whose paint arithmetic would spill 64 to 112 bytes a lane under those caps, keep
+// four-bucket groups and uncapped registers.
 template <int SPP, bool SRGB, bool GRAD>
-__global__ void __launch_bounds__(256) k_composite_ms(CovArgs A) {
+__global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 5 : 6))) k_composite_ms(CovArgs A) {
     constexpr int NW = Words<SPP>::n;
     constexpr int CPR = 64 / (2 * SPP);
     constexpr int NREG = 64 / CPR;
-    constexpr int G = 4;
+    constexpr int G = GRAD ? 4 : 2;
     constexpr uint32_t FULL = SPP == 8 ? 0xFFFFu : 0xFFFFFFFFu;  // both rows' bits (N = 32: each row)
     __shared__ float4 s_step[SRGB ? 256 : 1];
     __shared__ uint8_t s_lut[SRGB ? 4096 : 4];
