@@ -282,6 +282,25 @@ def test_wide_canvas_and_long_pieces(rasterizer):
     assert exact == 1.0
 
 
+def test_long_cubic_chains(rasterizer):
+    """Few paths whose cubics span the canvas: cubic solve chains of hundreds of cuts in the
+    longest length bins, where a wave's lanes each run a long sequential bisection chain;
+    every intersection t must equal the oracle's."""
+    W, H = 2048, 2048
+    rng = np.random.default_rng(0x5EED0C0B)
+    vg = VGContainer()
+    for k in range(24):
+        vg.addPath(k % 2, rgba(int(rng.integers(0, 256)), int(rng.integers(0, 256)), int(rng.integers(0, 256)),
+                               int(rng.integers(90, 256))))
+        pts = [tuple(float(v) for v in rng.uniform(-100.0, 2148.0, 2)) for _ in range(7)]
+        vg.addCubic(pts[0], pts[1], pts[2], pts[3])
+        vg.addCubic(pts[3], pts[4], pts[5], pts[6])
+        vg.addLinear(pts[6], pts[0])
+    cnt, exact = compare(rasterizer, vg, W, H, 32, label="long cubic chains")
+    assert cnt["curve_fragment"] > 20000
+    assert exact == 1.0
+
+
 def test_wide_sweep(rasterizer):
     scene = scanline.generate(scanline.scene.KIND_MIXED, 4500, 40, 60, 0x5EED0100)
     compare(rasterizer, scene, 4500, 40, 8, label="4500x40 mixed")
