@@ -106,23 +106,24 @@ __device__ __forceinline__ Arc arc(float2 c, float r, float a0, float a1) {
     return A;
 }
 
-// One curve of the path being built: vertices in the fixed order, and the per-curve words
-// the renderer reads.
-__device__ __forceinline__ void emit(const ChordScene &S, uint32_t c, uint32_t vp, uint32_t path, int nv,
+// One curve of the path being built, into the block's staging arrays (local curve cl and
+// vertex slot vl; the words hold the global vertex position vp and path): vertices in the
+// fixed order, and the per-curve words the renderer reads.
+__device__ __forceinline__ void emit(const ChordScene &S, uint32_t cl, uint32_t vl, uint32_t vp, uint32_t path, int nv,
                                      const float2 *p) {
     const uint32_t ixf = __float_as_uint(p[0].x), iyf = __float_as_uint(p[0].y);
     const uint32_t ixl = __float_as_uint(p[nv - 1].x), iyl = __float_as_uint(p[nv - 1].y);
     const bool rev = !(iyf < iyl || (iyf == iyl && ixf < ixl));
     for (int k = 0; k < nv; ++k) {
-        S.vtx[vp + k] = p[rev ? nv - 1 - k : k];
-        S.vpath[vp + k] = path;
+        S.vtx[vl + k] = p[rev ? nv - 1 - k : k];
+        S.vpath[vl + k] = path;
     }
-    S.ctype[c] = (uint8_t)nv;
-    S.cvpos[c] = vp;
-    S.cpath[c] = path;
-    S.cw[c] = -1.f;
-    S.crev[c] = rev ? 1 : 0;
-    S.cinfo[c] = make_uint4(vp, path, (uint32_t)nv | ((rev ? 1u : 0u) << 8), __float_as_uint(-1.f));
+    S.ctype[cl] = (uint8_t)nv;
+    S.cvpos[cl] = vp;
+    S.cpath[cl] = path;
+    S.cw[cl] = -1.f;
+    S.crev[cl] = rev ? 1 : 0;
+    S.cinfo[cl] = make_uint4(vp, path, (uint32_t)nv | ((rev ? 1u : 0u) << 8), __float_as_uint(-1.f));
 }
 
 __device__ __forceinline__ uint32_t fill_word(uint32_t c) { return (c & 0xFF000000u) ? c : 0u; }
@@ -132,7 +133,8 @@ struct ChordLaunch {
     ChordInterval d[kMaxChordDiagrams];
 };
 
-__device__ void gen_node(const ChordInterval &D, int i, uint32_t path, const ChordScene &S) {
+// Node i as path `path`, staged at local path pl.
+__device__ void gen_node(const ChordInterval &D, int i, uint32_t path, uint32_t pl, const ChordScene &S) {
     const float t = D.t;
     const float a0 = mix(D.node_t0_a[i], D.node_t0_b[i], t);
     const float a1 = mix(D.node_t1_a[i], D.node_t1_b[i], t);
@@ -143,20 +145,20 @@ __device__ void gen_node(const ChordInterval &D, int i, uint32_t path, const Cho
         in = arc(D.center, kInnerR, a0, a1);
         out = arc(D.center, kOuterR, a0, a1);
     }
-    S.frule[path] = 0;
-    S.finfo[path] = fill_word(D.node_color[i]);
-    const uint32_t c = 6u * path, v = 24u * path;
+    S.frule[pl] = 0;
+    S.finfo[pl] = fill_word(D.node_color[i]);
+    const uint32_t c = 6u * pl, v = 24u * pl, vg = 24u * path;
     float2 q[4];
     q[0] = out.p[0]; q[1] = in.p[0];
-    emit(S, c + 0, v + 0, path, 2, q);
+    emit(S, c + 0, v + 0, vg + 0, path, 2, q);
     q[0] = in.p[6]; q[1] = out.p[6];
-    emit(S, c + 1, v + 2, path, 2, q);
-    emit(S, c + 2, v + 4, path, 4, in.p);
-    emit(S, c + 3, v + 8, path, 4, in.p + 3);
+    emit(S, c + 1, v + 2, vg + 2, path, 2, q);
+    emit(S, c + 2, v + 4, vg + 4, path, 4, in.p);
+    emit(S, c + 3, v + 8, vg + 8, path, 4, in.p + 3);
     q[0] = out.p[6]; q[1] = out.p[5]; q[2] = out.p[4]; q[3] = out.p[3];
-    emit(S, c + 4, v + 12, path, 4, q);
+    emit(S, c + 4, v + 12, vg + 12, path, 4, q);
     q[0] = out.p[3]; q[1] = out.p[2]; q[2] = out.p[1]; q[3] = out.p[0];
-    emit(S, c + 5, v + 16, path, 4, q);
+    emit(S, c + 5, v + 16, vg + 16, path, 4, q);
     for (int k = 20; k < 24; ++k) {  // slots no curve uses
         S.vtx[v + k] = make_float2(-1000.f, -1000.f);
         S.vpath[v + k] = path;
@@ -165,7 +167,7 @@ __device__ void gen_node(const ChordInterval &D, int i, uint32_t path, const Cho
 
 // A link's ends are the stretches [pos, pos + size) of its nodes' sizes, mapped onto the
 // nodes' angles.  A node of size 0 maps them to its start (the reference divides by zero).
-__device__ void gen_link(const ChordInterval &D, int i, uint32_t path, const ChordScene &S) {
+__device__ void gen_link(const ChordInterval &D, int i, uint32_t path, uint32_t pl, const ChordScene &S) {
     const float t = D.t;
     const float4 fa = D.link_from[i], fb = D.link_to[i];  // {pos0, pos1, size0, size1}
     const float pos0 = mix(fa.x, fb.x, t), pos1 = mix(fa.y, fb.y, t);
@@ -191,36 +193,67 @@ __device__ void gen_link(const ChordInterval &D, int i, uint32_t path, const Cho
     const float2 c01 = make_float2(c.x + A.d6.x * rs, c.y + A.d6.y * rs);
     const float2 c10 = make_float2(c.x + B.d0.x * rs, c.y + B.d0.y * rs);
     const float2 c11 = make_float2(c.x + B.d6.x * rl, c.y + B.d6.y * rl);
-    S.frule[path] = 0;
-    S.finfo[path] = fill_word(D.node_color[sz0 > sz1 ? nd.x : nd.y] & 0x7FFFFFFFu);
-    const uint32_t cv = 6u * path, v = 24u * path;
+    S.frule[pl] = 0;
+    S.finfo[pl] = fill_word(D.node_color[sz0 > sz1 ? nd.x : nd.y] & 0x7FFFFFFFu);
+    const uint32_t cv = 6u * pl, v = 24u * pl, vg = 24u * path;
     float2 q[4];
     q[0] = A.p[6]; q[1] = c01; q[2] = c10; q[3] = B.p[0];
-    emit(S, cv + 0, v + 0, path, 4, q);
+    emit(S, cv + 0, v + 0, vg + 0, path, 4, q);
     q[0] = B.p[6]; q[1] = c11; q[2] = c00; q[3] = A.p[0];
-    emit(S, cv + 1, v + 4, path, 4, q);
-    emit(S, cv + 2, v + 8, path, 4, A.p);
-    emit(S, cv + 3, v + 12, path, 4, A.p + 3);
-    emit(S, cv + 4, v + 16, path, 4, B.p);
-    emit(S, cv + 5, v + 20, path, 4, B.p + 3);
+    emit(S, cv + 1, v + 4, vg + 4, path, 4, q);
+    emit(S, cv + 2, v + 8, vg + 8, path, 4, A.p);
+    emit(S, cv + 3, v + 12, vg + 12, path, 4, A.p + 3);
+    emit(S, cv + 4, v + 16, vg + 16, path, 4, B.p);
+    emit(S, cv + 5, v + 20, vg + 20, path, 4, B.p + 3);
 }
 
 // One thread per path; the diagrams' paths follow one another, each diagram's nodes before
-// its links (chord_animation, animation.cu:2582-2730).
-__global__ void __launch_bounds__(256) k_chord_frame(ChordLaunch L, uint32_t P, ChordScene S) {
-    const uint32_t path = blockIdx.x * blockDim.x + threadIdx.x;
-    if (path >= P) return;
-    uint32_t base = 0;
-    for (int d = 0; d < L.n_diagrams; ++d) {
-        const ChordInterval &D = L.d[d];
-        const uint32_t n = (uint32_t)D.n_nodes + (uint32_t)D.n_links;
-        if (path < base + n) {
-            const uint32_t i = path - base;
-            if (i < (uint32_t)D.n_nodes) gen_node(D, (int)i, path, S);
-            else gen_link(D, (int)(i - (uint32_t)D.n_nodes), path, S);
-            return;
+// its links (chord_animation, animation.cu:2582-2730).  A block stages its 64 paths' arrays
+// in LDS and then writes each array's contiguous stretch with consecutive lanes (a path's
+// own 24 vertices and 6 curves, written from one lane, would touch 64 lines a store).
+constexpr uint32_t kGenPaths = 64;
+
+__global__ void __launch_bounds__(64) k_chord_frame(ChordLaunch L, uint32_t P, ChordScene G) {
+    __shared__ float2 s_vtx[kGenPaths * 24];
+    __shared__ uint32_t s_vpath[kGenPaths * 24];
+    __shared__ uint4 s_cinfo[kGenPaths * 6];
+    __shared__ uint32_t s_cvpos[kGenPaths * 6], s_cpath[kGenPaths * 6], s_finfo[kGenPaths];
+    __shared__ float s_cw[kGenPaths * 6];
+    __shared__ uint8_t s_ctype[kGenPaths * 6], s_crev[kGenPaths * 6], s_frule[kGenPaths];
+    const ChordScene S{s_vtx, s_vpath, s_cvpos, s_cpath, s_finfo, s_ctype, s_frule, s_crev, s_cw, s_cinfo};
+    const uint32_t p0 = blockIdx.x * kGenPaths, pl = threadIdx.x, path = p0 + pl;
+    if (path < P) {
+        uint32_t base = 0;
+        for (int d = 0; d < L.n_diagrams; ++d) {
+            const ChordInterval &D = L.d[d];
+            const uint32_t n = (uint32_t)D.n_nodes + (uint32_t)D.n_links;
+            if (path < base + n) {
+                const uint32_t i = path - base;
+                if (i < (uint32_t)D.n_nodes) gen_node(D, (int)i, path, pl, S);
+                else gen_link(D, (int)(i - (uint32_t)D.n_nodes), path, pl, S);
+                break;
+            }
+            base += n;
         }
-        base += n;
+    }
+    __syncthreads();
+    const uint32_t np = P - p0 < kGenPaths ? P - p0 : kGenPaths;
+    for (uint32_t k = threadIdx.x; k < 24u * np; k += kGenPaths) {
+        G.vtx[24u * p0 + k] = s_vtx[k];
+        G.vpath[24u * p0 + k] = s_vpath[k];
+    }
+    for (uint32_t k = threadIdx.x; k < 6u * np; k += kGenPaths) {
+        const uint32_t c = 6u * p0 + k;
+        G.cinfo[c] = s_cinfo[k];
+        G.cvpos[c] = s_cvpos[k];
+        G.cpath[c] = s_cpath[k];
+        G.cw[c] = s_cw[k];
+        G.ctype[c] = s_ctype[k];
+        G.crev[c] = s_crev[k];
+    }
+    if (pl < np) {
+        G.frule[path] = s_frule[pl];
+        G.finfo[path] = s_finfo[pl];
     }
 }
 
@@ -241,7 +274,7 @@ void launch_chord_frame(const ChordInterval *d, int n_diagrams, uint32_t P, cons
     ChordLaunch L{};
     L.n_diagrams = n_diagrams;
     for (int i = 0; i < n_diagrams; ++i) L.d[i] = d[i];
-    if (P) hipLaunchKernelGGL(k_chord_frame, dim3((P + 255) / 256), dim3(256), 0, s, L, P, out);
+    if (P) hipLaunchKernelGGL(k_chord_frame, dim3((P + kGenPaths - 1) / kGenPaths), dim3(kGenPaths), 0, s, L, P, out);
 }
 
 }  // namespace sl
