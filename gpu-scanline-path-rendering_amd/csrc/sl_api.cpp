@@ -1068,6 +1068,9 @@ int sl_add_vg(sl_handle *h, const sl_scene *s) {
                           {&h->cpath, o_cpath, 4 * (size_t)C},   {&h->cw, o_cw, 4 * (size_t)C},
                           {&h->crev, o_crev, C},                 {&h->cinfo, o_cinfo, 16 * (size_t)C},
                           {&h->frule, o_frule, P},               {&h->finfo, o_finfo, 4 * (size_t)P}};
+    // from here on the old scene's arrays may be replaced: a failure leaves no scene
+    h->has_scene = false;
+    h->P = h->C = h->V = 0;
     for (const Part &q : parts) {
         int rc = ensure(h, *q.b, q.bytes);
         if (rc) return rc;
@@ -1121,9 +1124,10 @@ int sl_chord_load(sl_handle *h, const sl_chord *d, int n) {
             return fail(h, SL_E_ARG, "diagram %d: node array pointer is NULL", i);
         if (L && (!c.link_node || !c.link_from || !c.link_to))
             return fail(h, SL_E_ARG, "diagram %d: link array pointer is NULL", i);
-        for (uint32_t l = 0; l < 2 * (uint64_t)L; ++l)
+        for (uint64_t l = 0; l < 2 * (uint64_t)L; ++l)
             if (c.link_node[l] >= (uint32_t)c.n_nodes)
-                return fail(h, SL_E_ARG, "diagram %d: link %u names node %u >= n_nodes %d", i, l / 2, c.link_node[l], c.n_nodes);
+                return fail(h, SL_E_ARG, "diagram %d: link %llu names node %u >= n_nodes %d", i,
+                            (unsigned long long)(l / 2), c.link_node[l], c.n_nodes);
         paths += (uint64_t)c.n_nodes + most;
     }
     if (paths > (1u << 24)) return fail(h, SL_E_UNSUPPORTED, "chord scenes of more than 2^24 paths (%llu)", (unsigned long long)paths);
@@ -1150,7 +1154,12 @@ int sl_chord_load(sl_handle *h, const sl_chord *d, int n) {
         q.o_from = total; total += al(16 * L);
         q.o_to = total; total += al(16 * L);
     }
-    std::vector<uint8_t> blk(total);
+    std::vector<uint8_t> blk;
+    try {
+        blk.resize(total);
+    } catch (...) {  // no exception crosses the C ABI
+        return fail(h, SL_E_OOM, "host staging of %zu bytes of keyframes failed", total);
+    }
     for (int i = 0; i < n; ++i) {
         const sl_chord &c = d[i];
         const sl_handle::Chord &q = ch[i];
@@ -1227,6 +1236,9 @@ int sl_chord_frame(sl_handle *h, int time_ms, void *stream) {
         // a queued frame may still read the arrays about to be replaced
         int rc = settle(h);
         if (rc) return rc;
+        // the old scene's arrays may be replaced: a failure leaves no scene
+        h->has_scene = false;
+        h->P = h->C = h->V = 0;
         for (const Part &q : parts)
             if ((rc = ensure(h, *q.b, q.bytes))) return rc;
     }
