@@ -73,6 +73,22 @@ def test_chord_frame_renders_as_oracle(rasterizer, spp, ms):
         assert c["curve_fragment"] == ref.counts()["curve_fragment"]
 
 
+def test_chord_frame_multisample_32spp(rasterizer):
+    """The reference's default output (per-sample blending at 32 spp) of a generated frame,
+    against the OpenMP build of the oracle."""
+    ds = _diagrams(40, 120, seed=3)
+    r = rasterizer
+    _setup(r, 32, True)
+    anim = an.ChordAnimation(r, ds)
+    t = 8500
+    print("chord render: 32 spp multisample t =", t, flush=True)
+    anim.frame(t)
+    r.rasterize()
+    ref = pyoracle.run(pyoracle.chord_scene(pyoracle.chord_frame(ds, t)), W, H, 32, srgb=True, ms=True, keep=False,
+                       omp=True)
+    np.testing.assert_array_equal(r.output(), ref.get("image").reshape(H, W, 4))
+
+
 def test_full_chord_frame_renders_as_oracle(rasterizer):
     ds = _diagrams(150, 1500)
     r = rasterizer
