@@ -35,7 +35,7 @@ struct DevBuf {
 // arena whose arrays are carved out of the library's static scratch instead.
 struct Arena {
     DevBuf b_pcnt, b_nchain, b_choff, b_chains, b_chain_t;
-    DevBuf b_tvtx, b_pflags, b_prow, b_ccuts, b_cncut, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
+    DevBuf b_tvtx, b_pflags, b_prow, b_crec, b_cfrag, b_coff, b_ts, b_pnb, b_pbase, b_binfo, b_bcount,
         b_boff, b_bwsum, b_bcarry, b_rec_meta, b_rec_mask, b_scratch, b_items, b_imask, b_icov, b_bicnt, b_bmcnt,
         b_bext, b_big_list, b_small, b_nz_flag, b_nz_off, b_nzk0, b_nzk1, b_nzv0, b_nzv1, b_rs_hist, b_rs_hoff,
         b_rowbeg, b_rowend, b_rrec, b_btile, b_rtile, b_rpath, b_ptab, b_image, b_scan_tmp, b_lcount;
@@ -441,8 +441,8 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
     int rc2 = SL_OK;
     (void)rc2;
     if ((rc = ensure_in(h, A, A.b_tvtx, (size_t)V * 8)) || (rc = ensure_in(h, A, A.b_pflags, (size_t)P * 4)) ||
-        (rc = ensure_in(h, A, A.b_prow, (size_t)P * 8)) || (rc = ensure_in(h, A, A.b_ccuts, (size_t)C * 16)) ||
-        (rc = ensure_in(h, A, A.b_cncut, (size_t)C)) || (rc = ensure_in(h, A, A.b_cfrag, ((size_t)C + 1) * 4)) ||
+        (rc = ensure_in(h, A, A.b_prow, (size_t)P * 8)) || (rc = ensure_in(h, A, A.b_crec, (size_t)C * 64)) ||
+        (rc = ensure_in(h, A, A.b_cfrag, ((size_t)C + 1) * 4)) ||
         (rc = ensure_in(h, A, A.b_coff, ((size_t)C + 1) * 4)) || (rc = ensure_in(h, A, A.b_pnb, ((size_t)P + 1) * 4)) ||
         (rc = ensure_in(h, A, A.b_pcnt, (size_t)C * 20)) || (rc = ensure_in(h, A, A.b_nchain, ((size_t)C + 1) * 4)) ||
         (rc = ensure_in(h, A, A.b_choff, ((size_t)C + 1) * 4)) ||
@@ -452,7 +452,7 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
         (rc = ensure_in(h, A, A.b_scan_tmp, scan_need * 4)))
         return rc;
     f.tvtx = P_<float2>(A.b_tvtx); f.pflags = P_<uint32_t>(A.b_pflags); f.prow = P_<int2>(A.b_prow);
-    f.ccuts = P_<float4>(A.b_ccuts); f.cncut = P_<uint8_t>(A.b_cncut); f.cfrag = P_<uint32_t>(A.b_cfrag);
+    f.crec = P_<float4>(A.b_crec); f.cfrag = P_<uint32_t>(A.b_cfrag);
     f.coff = P_<uint32_t>(A.b_coff); f.pnb = P_<uint32_t>(A.b_pnb); f.pbase = P_<uint32_t>(A.b_pbase);
     f.pcnt = P_<uint32_t>(A.b_pcnt); f.nchain = P_<uint32_t>(A.b_nchain); f.choff = P_<uint32_t>(A.b_choff);
     f.image = carved ? out_image : P_<uint8_t>(A.b_image);
@@ -509,11 +509,11 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
         if (!carved) { h->last_F = 0; h->last_NCH = 0; h->last_NB = 0; h->sized = true; }
         return SL_OK;
     }
-    if ((rc = ensure_in(h, A, A.b_ts, (size_t)F * 8)) || (rc = ensure_in(h, A, A.b_chains, (size_t)NCH * 6 + 16)) ||
+    if ((rc = ensure_in(h, A, A.b_ts, (size_t)F * 8)) || (rc = ensure_in(h, A, A.b_chains, ((size_t)NCH + (size_t)NCH / 2 + 1) * 16)) ||
         (rc = ensure_in(h, A, A.b_chain_t, (size_t)F * 4 + 64)))  // padded: read in aligned 16-byte groups
         return rc;
     f.ts = P_<int2>(A.b_ts);
-    f.chains = P_<uint32_t>(A.b_chains);
+    f.chains = P_<uint4>(A.b_chains);
     f.plist = f.chains + (size_t)NCH;
     f.lcount = P_<uint32_t>(A.b_lcount);
     f.chain_t = P_<float>(A.b_chain_t);
@@ -769,7 +769,7 @@ int sl_destroy(sl_handle *h) {
     (void)hipDeviceSynchronize();
     DevBuf *bufs[] = {&h->A.b_pcnt, &h->A.b_nchain, &h->A.b_choff, &h->A.b_chains, &h->A.b_chain_t, &h->vtx, &h->vpath, &h->cvpos, &h->ctype, &h->cpath, &h->cw, &h->crev, &h->cinfo, &h->frule, &h->finfo,
                       &h->ptable[0], &h->ptable[1], &h->ptable[2], &h->ctab[0], &h->ctab[1], &h->A.b_tvtx, &h->A.b_pflags,
-                      &h->A.b_prow, &h->A.b_ccuts, &h->A.b_cncut, &h->A.b_cfrag, &h->A.b_coff, &h->A.b_ts, &h->A.b_pnb, &h->A.b_pbase,
+                      &h->A.b_prow, &h->A.b_crec, &h->A.b_cfrag, &h->A.b_coff, &h->A.b_ts, &h->A.b_pnb, &h->A.b_pbase,
                       &h->A.b_binfo, &h->A.b_bcount, &h->A.b_boff, &h->A.b_bwsum, &h->A.b_bcarry, &h->A.b_rec_meta,
                       &h->A.b_rec_mask, &h->A.b_scratch, &h->A.b_items, &h->A.b_imask, &h->A.b_icov, &h->A.b_bicnt, &h->A.b_bmcnt, &h->A.b_bext,
                       &h->A.b_big_list, &h->A.b_small, &h->A.b_nz_flag, &h->A.b_nz_off, &h->A.b_nzk0, &h->A.b_nzk1, &h->A.b_nzv0,

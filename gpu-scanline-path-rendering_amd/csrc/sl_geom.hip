@@ -210,6 +210,30 @@ struct Curve {
     float x0, x1, x2, x3, y0, y1, y2, y3;
 };
 
+// A curve and its monotone cuts from its 64-byte record (k_monotone writes it): one cache
+// line per curve for the intersection passes, whose work lists visit curves out of order.
+struct CurveRec {
+    Curve cu;
+    float4 q;       // monotone cuts
+    int n;          // cut count
+    uint32_t path;
+};
+
+__device__ __forceinline__ CurveRec load_rec(const float4 *__restrict__ crec, uint32_t c) {
+    CurveRec r;
+    const float4 a = crec[4 * (size_t)c], b = crec[4 * (size_t)c + 1];
+    r.q = crec[4 * (size_t)c + 2];
+    const float4 m = crec[4 * (size_t)c + 3];
+    const uint32_t tw = fb(m.x);
+    r.cu.type = (int)(tw & 0xFFu);
+    r.n = (int)((tw >> 8) & 0xFFu);
+    r.cu.w = m.y;
+    r.path = fb(m.z);
+    r.cu.x0 = a.x; r.cu.y0 = a.y; r.cu.x1 = a.z; r.cu.y1 = a.w;
+    r.cu.x2 = b.x; r.cu.y2 = b.y; r.cu.x3 = b.z; r.cu.y3 = b.w;
+    return r;
+}
+
 __device__ __forceinline__ Curve load_curve(const SceneDev &sc, const float2 *__restrict__ tv, uint32_t c) {
     Curve cu;
     const uint4 ci = sc.cinfo[c];
@@ -320,67 +344,79 @@ __device__ __forceinline__ void curve_rows(const Curve &cu, int H, int &rlo, int
 }
 
 __global__ void __launch_bounds__(256) k_monotone(SceneDev sc, const float2 *__restrict__ tv, const uint32_t *__restrict__ pflags,
-                                                   int W, int H, float4 *ccuts, uint8_t *cncut, uint32_t *cfrag,
+                                                   int W, int H, float4 *crec, uint32_t *cfrag,
                                                    uint32_t *pcnt, uint32_t *nchain, int2 *prow) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= sc.C) return;
-    Curve cu = load_curve(sc, tv, c);
-    const bool vis = region_visible(pflags[sc.cpath[c]]);
-    path_canvas(sc, sc.cpath[c], W, H);
-    float tq[4] = {0.f, 0.f, 0.f, 0.f};
-    int n = 0;
-    if (vis) {
-        monotone_axis(cu, cu.x0, cu.x1, cu.x2, cu.x3, tq, n);
-        monotone_axis(cu, cu.y0, cu.y1, cu.y2, cu.y3, tq, n);
-    }
-    float q0 = tq[0], q1 = tq[1], q2 = tq[2], q3 = tq[3];
-    if (n >= 2) { if (q1 < q0) { float t = q0; q0 = q1; q1 = t; } }
-    if (n >= 3) {
-        float t2 = q2, t1 = q1;
-        if (t2 < t1) {
-            q2 = t1;
-            float t0 = q0;
-            if (t2 < t0) { q1 = t0; q0 = t2; } else { q1 = t2; }
+    // the block's 64-byte curve records, staged so that they are stored with consecutive lanes
+    __shared__ float4 stg[256 * 4];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = c < sc.C;
+    bool vis = false;
+    int rlo = 0, rhi = -1;
+    uint32_t path = 0u;
+    if (live) {
+        Curve cu = load_curve(sc, tv, c);
+        path = sc.cpath[c];
+        vis = region_visible(pflags[path]);
+        path_canvas(sc, path, W, H);
+        float tq[4] = {0.f, 0.f, 0.f, 0.f};
+        int n = 0;
+        if (vis) {
+            monotone_axis(cu, cu.x0, cu.x1, cu.x2, cu.x3, tq, n);
+            monotone_axis(cu, cu.y0, cu.y1, cu.y2, cu.y3, tq, n);
         }
-    }
-    if (n >= 4) {
-        float t3 = q3, t2 = q2;
-        if (t3 < t2) {
-            q3 = t2;
-            float t1 = q1;
-            if (t3 < t1) {
+        float q0 = tq[0], q1 = tq[1], q2 = tq[2], q3 = tq[3];
+        if (n >= 2) { if (q1 < q0) { float t = q0; q0 = q1; q1 = t; } }
+        if (n >= 3) {
+            float t2 = q2, t1 = q1;
+            if (t2 < t1) {
                 q2 = t1;
                 float t0 = q0;
-                if (t3 < t0) { q1 = t0; q0 = t3; } else { q1 = t3; }
-            } else {
-                q2 = t3;
+                if (t2 < t0) { q1 = t0; q0 = t2; } else { q1 = t2; }
             }
         }
-    }
-    ccuts[c] = make_float4(q0, q1, q2, q3);
-    cncut[c] = (uint8_t)n;
-    uint32_t cnt = 0;
-    uint32_t pc[5] = {0u, 0u, 0u, 0u, 0u};
-    const int npieces = vis ? n + 1 : 0;
-    if (vis) {
-        float v0x = cu.x0, v0y = cu.y0;
-        for (int ms = 0; ms < npieces; ++ms) {
-            float t1 = (ms == n) ? 1.f : pick5(ms, q0, q1, q2, q3, 1.f);
-            float v1x = ex(cu, t1), v1y = ey(cu, t1);
-            CutRange r = cut_range(v0x, v0y, v1x, v1y, W, H);
-            cnt += 1u + (uint32_t)r.nx + (uint32_t)r.ny;
-            pc[ms] = (uint32_t)r.nx | ((uint32_t)r.ny << 16);
-            v0x = v1x; v0y = v1y;
+        if (n >= 4) {
+            float t3 = q3, t2 = q2;
+            if (t3 < t2) {
+                q3 = t2;
+                float t1 = q1;
+                if (t3 < t1) {
+                    q2 = t1;
+                    float t0 = q0;
+                    if (t3 < t0) { q1 = t0; q0 = t3; } else { q1 = t3; }
+                } else {
+                    q2 = t3;
+                }
+            }
         }
-    }
+        stg[4 * threadIdx.x] = make_float4(cu.x0, cu.y0, cu.x1, cu.y1);
+        stg[4 * threadIdx.x + 1] = make_float4(cu.x2, cu.y2, cu.x3, cu.y3);
+        stg[4 * threadIdx.x + 2] = make_float4(q0, q1, q2, q3);
+        stg[4 * threadIdx.x + 3] = make_float4(bf((uint32_t)cu.type | ((uint32_t)n << 8)), cu.w, bf(path), 0.f);
+        uint32_t cnt = 0;
+        uint32_t pc[5] = {0u, 0u, 0u, 0u, 0u};
+        const int npieces = vis ? n + 1 : 0;
+        if (vis) {
+            float v0x = cu.x0, v0y = cu.y0;
+            for (int ms = 0; ms < npieces; ++ms) {
+                float t1 = (ms == n) ? 1.f : pick5(ms, q0, q1, q2, q3, 1.f);
+                float v1x = ex(cu, t1), v1y = ey(cu, t1);
+                CutRange r = cut_range(v0x, v0y, v1x, v1y, W, H);
+                cnt += 1u + (uint32_t)r.nx + (uint32_t)r.ny;
+                pc[ms] = (uint32_t)r.nx | ((uint32_t)r.ny << 16);
+                v0x = v1x; v0y = v1y;
+            }
+        }
 #pragma unroll
-    for (int k = 0; k < 5; ++k) pcnt[5 * (size_t)c + k] = pc[k];
-    cfrag[c] = cnt;
-    nchain[c] = 2u * (uint32_t)npieces;
+        for (int k = 0; k < 5; ++k) pcnt[5 * (size_t)c + k] = pc[k];
+        cfrag[c] = cnt;
+        nchain[c] = 2u * (uint32_t)npieces;
+        if (vis) curve_rows(cu, H, rlo, rhi);
+    }
+    __syncthreads();
+    const uint32_t c0 = blockIdx.x * blockDim.x;
+    const uint32_t nrec = 4u * (sc.C - c0 < blockDim.x ? sc.C - c0 : blockDim.x);
+    for (uint32_t k = threadIdx.x; k < nrec; k += blockDim.x) crec[4 * (size_t)c0 + k] = stg[k];
     // the path's row range, one atomic pair per distinct path of the wave
-    int rlo = 0, rhi = -1;
-    if (vis) curve_rows(cu, H, rlo, rhi);
-    const uint32_t path = sc.cpath[c];
     const int lane = threadIdx.x & 63;
     uint64_t pend = __ballot(vis && rlo <= rhi);
     while (pend) {
@@ -483,8 +519,9 @@ __global__ void __launch_bounds__(kChainListThreads) k_list_count(SceneDev sc, u
 __global__ void __launch_bounds__(kChainListThreads) k_list_place(SceneDev sc, uint32_t C,
                                                                   const uint32_t *__restrict__ nchain,
                                                                   const uint32_t *__restrict__ pcnt,
+                                                                  const uint32_t *__restrict__ coff,
                                                                   const uint32_t *__restrict__ cnt, uint32_t *cursor,
-                                                                  uint32_t *clist, uint32_t *plist) {
+                                                                  uint4 *clist, uint4 *plist) {
     __shared__ uint32_t h[kListClasses], base[kListClasses];
     for (int q = threadIdx.x; q < kListClasses; q += blockDim.x) h[q] = 0u;
     if (threadIdx.x == 0) {
@@ -511,14 +548,19 @@ __global__ void __launch_bounds__(kChainListThreads) k_list_place(SceneDev sc, u
     for (int q = threadIdx.x; q < kListClasses; q += blockDim.x)
         if (h[q]) base[q] += atomicAdd(&cursor[q], h[q]);
     __syncthreads();
+    uint32_t off = cp.np ? coff[c] : 0u;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (k >= cp.np) break;
         int qx, qy, qp;
         piece_classes(cp, k, qx, qy, qp);
-        if (qx >= 0) clist[base[qx] + rk[3 * k]] = (c << 4) | ((uint32_t)k << 1);
-        if (qy >= 0) clist[base[qy] + rk[3 * k + 1]] = (c << 4) | ((uint32_t)k << 1) | 1u;
-        plist[base[qp] + rk[3 * k + 2]] = (c << 3) | (uint32_t)k;
+        // each entry carries its piece's ts offset and cut counts, so that the intersection
+        // passes read only the curve's record besides it
+        const uint32_t pb = off, pw = cp.pc[k];
+        off += 1u + (pw & 0xFFFFu) + (pw >> 16);
+        if (qx >= 0) clist[base[qx] + rk[3 * k]] = make_uint4((c << 4) | ((uint32_t)k << 1), pb, pw, 0u);
+        if (qy >= 0) clist[base[qy] + rk[3 * k + 1]] = make_uint4((c << 4) | ((uint32_t)k << 1) | 1u, pb, pw, 0u);
+        plist[base[qp] + rk[3 * k + 2]] = make_uint4((c << 3) | (uint32_t)k, pb, pw, 0u);
     }
 }
 
@@ -590,22 +632,19 @@ struct PieceSetup {
 };
 
 // piece k of curve c: its ts offset, cut counts, start flags and both sides' solvers
-__device__ __forceinline__ PieceSetup piece_setup(const SceneDev &sc, const float2 *__restrict__ tv,
-                                                  const float4 *__restrict__ ccuts, const uint8_t *__restrict__ cncut,
-                                                  const uint32_t *__restrict__ coff, const uint32_t *__restrict__ pcnt,
-                                                  uint32_t c, int k, int W, int H) {
+// (e: the work-list entry {curve and piece, ts offset of the piece, nx | ny << 16})
+__device__ __forceinline__ PieceSetup piece_setup(const SceneDev &sc, const float4 *__restrict__ crec,
+                                                  uint32_t c, int k, uint4 e, int W, int H) {
     PieceSetup p;
     p.c = c;
-    const uint32_t *pc = pcnt + 5 * (size_t)c;
-    uint32_t off = coff[c];
-    for (int kk = 0; kk < k; ++kk) off += 1u + (pc[kk] & 0xFFFFu) + (pc[kk] >> 16);
-    p.base = off;
-    p.nx = pc[k] & 0xFFFFu;
-    p.ny = pc[k] >> 16;
-    path_canvas(sc, sc.cpath[c], W, H);
-    const Curve cu = load_curve(sc, tv, c);
-    const float4 q = ccuts[c];
-    const int n = cncut[c];
+    p.base = e.y;
+    p.nx = e.z & 0xFFFFu;
+    p.ny = e.z >> 16;
+    const CurveRec rec = load_rec(crec, c);
+    path_canvas(sc, rec.path, W, H);
+    const Curve &cu = rec.cu;
+    const float4 q = rec.q;
+    const int n = rec.n;
     float v0x = cu.x0, v0y = cu.y0, t0m = 0.f;
     if (k > 0) {
         const float tr = pick5(k - 1, q.x, q.y, q.z, q.w, 1.f);
@@ -635,12 +674,11 @@ __device__ __forceinline__ PieceSetup piece_setup(const SceneDev &sc, const floa
 
 // Intersection pass 1, long pieces: one thread per (curve, piece, side) chain, the t's
 // stored for the merge pass (k_merge_long) with 16-byte stores.
-__device__ __forceinline__ void solve_chain(const SceneDev &sc, const float2 *__restrict__ tv,
-                                            const float4 *__restrict__ ccuts, const uint8_t *__restrict__ cncut,
-                                            const uint32_t *__restrict__ coff, const uint32_t *__restrict__ pcnt,
-                                            uint32_t d, int W, int H, float *chain_t) {
+__device__ __forceinline__ void solve_chain(const SceneDev &sc, const float4 *__restrict__ crec, uint4 e, int W,
+                                            int H, float *chain_t) {
+    const uint32_t d = e.x;
     const int side = (int)(d & 1u);
-    PieceSetup p = piece_setup(sc, tv, ccuts, cncut, coff, pcnt, d >> 4, (int)((d >> 1) & 7u), W, H);
+    PieceSetup p = piece_setup(sc, crec, d >> 4, (int)((d >> 1) & 7u), e, W, H);
     const uint32_t cnt = side ? p.ny : p.nx;
     SideSolver sv = side ? p.sy : p.sx;
     float *out = chain_t + p.base + (side ? p.nx : 0u);
@@ -679,24 +717,19 @@ struct PieceRef {
     float t0m;
 };
 
-__device__ __forceinline__ PieceRef piece_ref(const SceneDev &sc, const float2 *__restrict__ tv,
-                                              const float4 *__restrict__ ccuts, const uint32_t *__restrict__ coff,
-                                              const uint32_t *__restrict__ pcnt, uint32_t d) {
+__device__ __forceinline__ PieceRef piece_ref(const float4 *__restrict__ crec, uint4 e) {
     PieceRef r;
+    const uint32_t d = e.x;
     r.c = d >> 3;
     const int k = (int)(d & 7u);
-    const uint32_t *pc = pcnt + 5 * (size_t)r.c;
-    uint32_t p = 0;
-    for (int kk = 0; kk < k; ++kk) p += 1u + (pc[kk] & 0xFFFFu) + (pc[kk] >> 16);
-    r.base = coff[r.c] + p;
-    r.nx = pc[k] & 0xFFFFu;
-    r.ny = pc[k] >> 16;
+    r.base = e.y;
+    r.nx = e.z & 0xFFFFu;
+    r.ny = e.z >> 16;
     r.t0m = 0.f;
     if (k > 0) {
-        const Curve cu = load_curve(sc, tv, r.c);
-        const float4 q = ccuts[r.c];
-        const float tr = pick5(k - 1, q.x, q.y, q.z, q.w, 1.f);
-        r.t0m = piece_end_flags(tr, ex(cu, tr));
+        const CurveRec rec = load_rec(crec, r.c);
+        const float tr = pick5(k - 1, rec.q.x, rec.q.y, rec.q.z, rec.q.w, 1.f);
+        r.t0m = piece_end_flags(tr, ex(rec.cu, tr));
     }
     return r;
 }
@@ -830,15 +863,13 @@ __device__ __forceinline__ void walk_pieces_staged(PieceSetup &p, bool live, int
 // Intersection pass 1.  Blocks [0, chain_blocks) solve the listed chains of long pieces,
 // longest first; the blocks after them each take 256 short pieces and walk them while
 // solving, so the short pieces fill the GPU while the long chains run.
-__global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__restrict__ tv, const float4 *__restrict__ ccuts,
-                                                 const uint8_t *__restrict__ cncut, const uint32_t *__restrict__ coff,
-                                                 const uint32_t *__restrict__ pcnt, const uint32_t *__restrict__ clist,
-                                                 const uint32_t *__restrict__ plist, const uint32_t *__restrict__ cnt,
+__global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float4 *__restrict__ crec, const uint4 *__restrict__ clist,
+                                                 const uint4 *__restrict__ plist, const uint32_t *__restrict__ cnt,
                                                  uint32_t chain_blocks, int W, int H, float *chain_t, int2 *ts) {
     if (blockIdx.x < chain_blocks) {
         const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
         if (i >= class_total(cnt, 0, kChainClasses)) return;
-        solve_chain(sc, tv, ccuts, cncut, coff, pcnt, clist[i], W, H, chain_t);
+        solve_chain(sc, crec, clist[i], W, H, chain_t);
         return;
     }
     __shared__ uint32_t stage[4][kStageSteps * 64];
@@ -847,8 +878,8 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
     if (!__ballot(live)) return;  // whole wave past the list
     PieceSetup p;
     if (live) {
-        const uint32_t d = plist[cnt[kLongPieceClass] + g];
-        p = piece_setup(sc, tv, ccuts, cncut, coff, pcnt, d >> 3, (int)(d & 7u), W, H);
+        const uint4 e = plist[cnt[kLongPieceClass] + g];
+        p = piece_setup(sc, crec, e.x >> 3, (int)(e.x & 7u), e, W, H);
     }
     walk_pieces_staged(p, live, ts, stage[threadIdx.x >> 6]);
 }
@@ -860,17 +891,15 @@ __global__ void __launch_bounds__(256) k_chains(SceneDev sc, const float2 *__res
 // Runs of equal t then take the OR of their flags up to the next entry of the run.
 constexpr int kMergeCap = 1024;
 
-__global__ void __launch_bounds__(64) k_merge_long(SceneDev sc, const float2 *__restrict__ tv,
-                                                   const float4 *__restrict__ ccuts, const uint32_t *__restrict__ coff,
-                                                   const uint32_t *__restrict__ pcnt, const float *__restrict__ chain_t,
-                                                   const uint32_t *__restrict__ plist, const uint32_t *__restrict__ counts,
+__global__ void __launch_bounds__(64) k_merge_long(const float4 *__restrict__ crec, const float *__restrict__ chain_t,
+                                                   const uint4 *__restrict__ plist, const uint32_t *__restrict__ counts,
                                                    int2 *ts) {
     __shared__ float sa[kMergeCap], sb[kMergeCap];
     __shared__ uint32_t sm[kMergeCap];
     const int lane = threadIdx.x;
     const uint32_t n0 = counts[0];
     for (uint32_t g = blockIdx.x; g < n0; g += gridDim.x) {
-        const PieceRef r = piece_ref(sc, tv, ccuts, coff, pcnt, plist[g]);
+        const PieceRef r = piece_ref(crec, plist[g]);
         const uint32_t nx = r.nx, ny = r.ny, n = nx + ny + 1;
         if (n > (uint32_t)kMergeCap) {
             // an over-long piece: the walk itself
@@ -982,7 +1011,7 @@ void launch_transform(const SceneDev &sc, Frame &f, hipStream_t s) {
 void launch_monotone(const SceneDev &sc, Frame &f, hipStream_t s) {
     if (!sc.C) return;
     hipLaunchKernelGGL(k_monotone, dim3(blocks_for(sc.C, 256)), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                       (const uint32_t *)f.pflags, f.W, f.H, f.ccuts, f.cncut, f.cfrag, f.pcnt, f.nchain, f.prow);
+                       (const uint32_t *)f.pflags, f.W, f.H, f.crec, f.cfrag, f.pcnt, f.nchain, f.prow);
 }
 
 void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s) {
@@ -992,22 +1021,20 @@ void launch_intersect(const SceneDev &sc, Frame &f, hipStream_t s) {
     const dim3 gl(blocks_for(sc.C, kChainListThreads)), bl(kChainListThreads);
     hipLaunchKernelGGL(k_list_count, gl, bl, 0, s, sc, sc.C, (const uint32_t *)f.nchain, (const uint32_t *)f.pcnt, cnt);
     hipLaunchKernelGGL(k_list_place, gl, bl, 0, s, sc, sc.C, (const uint32_t *)f.nchain, (const uint32_t *)f.pcnt,
-                       (const uint32_t *)cnt, cursor, f.chains, f.plist);
+                       (const uint32_t *)f.coff, (const uint32_t *)cnt, cursor, f.chains, f.plist);
     // chain blocks: enough for every chain; piece blocks: enough for every piece
     const uint32_t npieces = (uint32_t)(f.NCH / 2);
     const uint32_t chain_blocks = blocks_for(f.NCH, 256), piece_blocks = blocks_for(npieces, 256);
     if (f.NCH)
-        hipLaunchKernelGGL(k_chains, dim3(chain_blocks + piece_blocks), dim3(256), 0, s, sc, (const float2 *)f.tvtx,
-                           (const float4 *)f.ccuts, (const uint8_t *)f.cncut, (const uint32_t *)f.coff,
-                           (const uint32_t *)f.pcnt, (const uint32_t *)f.chains, (const uint32_t *)f.plist,
+        hipLaunchKernelGGL(k_chains, dim3(chain_blocks + piece_blocks), dim3(256), 0, s, sc, (const float4 *)f.crec,
+                           (const uint4 *)f.chains, (const uint4 *)f.plist,
                            (const uint32_t *)cnt, chain_blocks, f.W, f.H, f.chain_t, f.ts);
     // one block per long piece up to 32768 blocks: the pieces' sizes vary widely, and
     // blocks that find no piece exit at once
     if (npieces)
-        hipLaunchKernelGGL(k_merge_long, dim3(npieces < 32768u ? npieces : 32768u), dim3(64), 0, s, sc,
-                           (const float2 *)f.tvtx, (const float4 *)f.ccuts,
-                           (const uint32_t *)f.coff, (const uint32_t *)f.pcnt, (const float *)f.chain_t,
-                           (const uint32_t *)f.plist, (const uint32_t *)(cnt + kLongPieceClass), f.ts);
+        hipLaunchKernelGGL(k_merge_long, dim3(npieces < 32768u ? npieces : 32768u), dim3(64), 0, s,
+                           (const float4 *)f.crec, (const float *)f.chain_t, (const uint4 *)f.plist,
+                           (const uint32_t *)(cnt + kLongPieceClass), f.ts);
 }
 
 }  // namespace sl

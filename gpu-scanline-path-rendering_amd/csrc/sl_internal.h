@@ -84,15 +84,19 @@ struct Frame {
     float2 *tvtx;            // [V]
     uint32_t *pflags;        // [P] 8 region bits
     int2 *prow;              // [P] min / max valid cell row
-    float4 *ccuts;           // [C]
-    uint8_t *cncut;          // [C]
+    float4 *crec;            // [4C] per curve, one 64-byte record read by the intersection
+                             //   passes: {x0, y0, x1, y1}, {x2, y2, x3, y3} (transformed),
+                             //   the monotone cuts {t0..t3}, {type | cut count << 8, weight
+                             //   bits, path, 0}
     uint32_t *cfrag;         // [C+1]  -> exclusive offsets after scan
     uint32_t *coff;          // [C+1]
     uint32_t *pcnt;          // [5C] per monotone piece: x cuts | y cuts << 16
     uint32_t *nchain;        // [C+1] solve chains per curve (2 per piece)
     uint32_t *choff;         // [C+1] exclusive scan of nchain
-    uint32_t *chains;        // [NCH] solve chains of long pieces, grouped by class
-    uint32_t *plist;         // [NCH/2] pieces: long ones, then short ones grouped by class
+    uint4 *chains;           // [NCH] solve chains of long pieces, grouped by class, and
+    uint4 *plist;            // [NCH/2] pieces: long ones, then short ones grouped by class;
+                             //   entries {curve << 4 | piece << 1 | side (chains) or curve << 3 |
+                             //   piece, ts offset of the piece, nx | ny << 16, 0}
     uint32_t *lcount;        // [2 * 101] class counts (50 chain, 1 long-piece, 50 short-piece
                              //   classes), then the place pass's cursors
     int64_t NCH;
