@@ -1340,18 +1340,41 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 6) k_comp
 // (column p * CPR + q, row 2r + dy) of the 64x2 tile in state register p, CPR = 64 / 2N
 // columns per register.  Each register that holds a pixel covered by the bucket costs one
 // wave-wide blend (its lanes select by their sample bits), so a 2x2 fragment or a span
-// column costs the same whatever its sample count, and no lane loops over samples.  The
-// register index is wave-uniform, so the indexed state stays in VGPRs (s_set_gpr_idx).
+// column costs the same whatever its sample count, and no lane loops over samples.  Solid
+// paints run over the state registers in static blocks of four (apply_blocks): a
+// register's blend is four byte lookups in the path's table and one select whose lane mask
+// is read straight from the column lanes' sample bits, with no register indexing.
 // ---------------------------------------------------------------------------------
 typedef uint32_t u32x32 __attribute__((ext_vector_type(32)));
 typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
 
-// Runs f(state, p) -> new state on every register p of the set bits of m (p = pbase + bit).
-// The 4-way form takes four registers an iteration so that their LDS round trips overlap;
-// missing ones repeat the first register, whose duplicate writes store the same value, so
-// every write is unconditional.  A write that depends on control flow inside the loop
-// makes the compiler copy the whole 32-register vector at every iteration, and so does a
-// loop that writes two vectors: each call covers one.
+// Runs f(state, p) -> new state on every register of the BLK-register blocks of vec that
+// hold a set bit of m (p = pbase + register), with static register numbers: the state
+// stays in plain VGPRs (no indexing, no per-register loop), and a block's LDS round trips
+// overlap, at the price of the block's uncovered registers (f leaves a register it does not
+// cover unchanged).
+constexpr int kMsBlock = 4;
+template <int N, int BLK, class V, class F>
+__device__ __forceinline__ void apply_blocks(V &vec, uint32_t m, int pbase, const F &f) {
+#pragma unroll
+    for (int b0 = 0; b0 < N; b0 += BLK) {
+        if ((m >> b0) & ((1u << BLK) - 1u)) {
+#pragma unroll
+            for (int i = 0; i < BLK; ++i) vec[b0 + i] = f(vec[b0 + i], pbase + b0 + i);
+        }
+    }
+}
+
+// Runs f(state, p) -> new state on every register p of the set bits of m (p = pbase + bit)
+// by index: the register number is wave-uniform, so the indexed state stays in VGPRs
+// (s_set_gpr_idx).  The gradient variants use these, since their paint arithmetic per
+// register is large (the static blocks would spend it on uncovered registers too) and a
+// function that mixes indexed and static access to the state spills it.  The 4-way form
+// takes four registers an iteration so that their LDS round trips overlap; missing ones
+// repeat the first register, whose duplicate writes store the same value, so every write
+// is unconditional.  A write that depends on control flow inside the loop makes the
+// compiler copy the whole 32-register vector at every iteration, and so does a loop that
+// writes two vectors: each call covers one.
 template <class V, class F>
 __device__ __forceinline__ void apply_regs4(V &vec, uint32_t m, int pbase, const F &f) {
     while (m) {
@@ -1390,6 +1413,10 @@ template <> struct SampleRegs<64> {
         for (int i = 0; i < 32; ++i) { a[i] = v; b[i] = v; }
     }
     __device__ __forceinline__ uint32_t get(int p) const { return p < 32 ? a[p] : b[p - 32]; }
+    template <int BLK, class F> __device__ __forceinline__ void apply(uint64_t m, const F &f) {
+        apply_blocks<32, BLK>(a, (uint32_t)m, 0, f);
+        apply_blocks<32, BLK>(b, (uint32_t)(m >> 32), 32, f);
+    }
     template <class F> __device__ __forceinline__ void apply4(uint64_t m, const F &f) {
         apply_regs4(a, (uint32_t)m, 0, f);
         apply_regs4(b, (uint32_t)(m >> 32), 32, f);
@@ -1406,6 +1433,9 @@ template <> struct SampleRegs<32> {
         for (int i = 0; i < 32; ++i) a[i] = v;
     }
     __device__ __forceinline__ uint32_t get(int p) const { return a[p]; }
+    template <int BLK, class F> __device__ __forceinline__ void apply(uint64_t m, const F &f) {
+        apply_blocks<32, BLK>(a, (uint32_t)m, 0, f);
+    }
     template <class F> __device__ __forceinline__ void apply4(uint64_t m, const F &f) { apply_regs4(a, (uint32_t)m, 0, f); }
     template <class F> __device__ __forceinline__ void apply1(uint64_t m, const F &f) { apply_regs1(a, (uint32_t)m, 0, f); }
 };
@@ -1416,6 +1446,9 @@ template <> struct SampleRegs<16> {
         for (int i = 0; i < 16; ++i) a[i] = v;
     }
     __device__ __forceinline__ uint32_t get(int p) const { return a[p]; }
+    template <int BLK, class F> __device__ __forceinline__ void apply(uint64_t m, const F &f) {
+        apply_blocks<16, BLK>(a, (uint32_t)m, 0, f);
+    }
     template <class F> __device__ __forceinline__ void apply4(uint64_t m, const F &f) { apply_regs4(a, (uint32_t)m, 0, f); }
     template <class F> __device__ __forceinline__ void apply1(uint64_t m, const F &f) { apply_regs1(a, (uint32_t)m, 0, f); }
 };
@@ -1463,8 +1496,9 @@ __global__ void __launch_bounds__(256) k_paint_tables(uint32_t P, const uint32_t
 // A code's blend through a planar table: channel ch's byte indexes bytes ch * 256 .. +255.
 __device__ __forceinline__ uint32_t table_code(const uint32_t *T, uint32_t st) {
     const uint8_t *b = (const uint8_t *)T;
-    return (uint32_t)b[st & 0xFFu] | ((uint32_t)b[256 + ((st >> 8) & 0xFFu)] << 8) |
-           ((uint32_t)b[512 + ((st >> 16) & 0xFFu)] << 16) | ((uint32_t)b[768 + (st >> 24)] << 24);
+    const uint32_t lo = (uint32_t)b[st & 0xFFu] | ((uint32_t)b[256 + ((st >> 8) & 0xFFu)] << 8);
+    const uint32_t hi = (uint32_t)b[512 + ((st >> 16) & 0xFFu)] | ((uint32_t)b[768 + (st >> 24)] << 8);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);  // the low half-words of lo and hi
 }
 
 // One sample's framebuffer code blended with paint src (decoded colour, alpha a) and
@@ -1523,13 +1557,15 @@ __device__ __forceinline__ uint32_t column_bits(const uint32_t *m, int dx, uint3
     return (m[0] >> (16 * dx)) & 0xFFFFu;
 }
 
-// Occupancy over group size: two buckets a group (half the LDS and prefetch registers of
-// four) with registers capped for 4 / 5 / 6 waves per SIMD at 32 / 16 / 8 samples runs the
-// headline scene's multisample composite in 13.9 / 7.4 / 4.6 ms against 16.1 / 8.1 / 5.7 ms
-// for four-bucket groups at 3 / 4 / 5 waves (at 32 samples the cap spills 32 bytes a lane,
-// still faster than one-bucket groups without spills, 14.5 ms).  The gradient variants,
-// whose paint arithmetic would spill 64 to 112 bytes a lane under those caps, keep
-// four-bucket groups and uncapped registers.
+// Measured on the headline scene's multisample composite at 32 / 16 / 8 samples: the
+// static blocks take 10.0 / 5.7 / 4.6 ms where the indexed four-register loop took
+// 13.9 / 7.4 / 4.6 ms (per register it spent 14 VALU, 9 scalar and 5 LDS instructions
+// against 10 VALU and 4 LDS, the blocks' idle registers included).  Blocks of two or eight
+// registers, one- or four-bucket groups, and three waves per SIMD at 32 samples were all
+// slower (10.2 / 10.3 / 10.6 / 10.5 / 11.7 ms).  Two buckets a group with registers capped
+// for 4 / 5 / 6 waves per SIMD at 32 / 16 / 8 samples (16 bytes a lane spilled at 32).  The
+// gradient variants, whose paint arithmetic would spill under those caps, keep four-bucket
+// groups, uncapped registers and the indexed loops.
 template <int SPP, bool SRGB, bool GRAD>
 __global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 5 : 6))) k_composite_ms(CovArgs A) {
     constexpr int NW = Words<SPP>::n;
@@ -1546,7 +1582,6 @@ __global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 
     __shared__ uint32_t l_pid[4][kCovList];
     __shared__ uint32_t slot[4][G][64];
     __shared__ uint32_t tbl[4][G][256];
-    __shared__ uint2 cmask[4][G][64];   // per column: both rows' sample bits (N = 32), or packed
     for (int i = threadIdx.x; i < 256; i += 256) {
         if (SRGB) s_step[i] = A.step_c[i];
         s_dec[i] = A.dec_c[i];
@@ -1565,7 +1600,7 @@ __global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 
     const int x = tx0 + lane;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // blend-phase coordinates of this lane
-    const int q = lane / (2 * SPP), dy = (lane / SPP) & 1, smp = lane & (SPP - 1);
+    const int q = lane / (2 * SPP), dy = (lane / SPP) & 1;
     SampleRegs<NREG> S;
     S.init(0xFFFFFFFFu);
 
@@ -1636,9 +1671,7 @@ __global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 
                     cbh[g] = SPP == 32 ? hi : 0u;
                     if (GRAD) ce[g] = (uint32_t)px0 | ((uint32_t)(pw ? pw : 2) << 16);
                 }
-                cmask[w][g][lane] = make_uint2(cb[g], cbh[g]);
             }
-            wave_sync_lds();
             // blend, bucket by bucket in path order, one wave-wide pass per state register
             // holding a covered column: four LDS reads of the path's blend table for a
             // solid paint, the blend arithmetic on the pixel's own paint for a gradient.
@@ -1650,17 +1683,31 @@ __global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 
                 const float4 src = l_src[w][g0 + g];
                 const bool gpaint = GRAD && src.w < 0.f;
                 const uint32_t ceg = GRAD ? ce[g] : 0u;
-                // this lane's sample bit under register p's covering item: its column's
-                // masks from LDS (N = 32: the row's word; else both rows packed)
-                const uint32_t *cmw = (const uint32_t *)&cmask[w][g][0];
-                const int moff = SPP == 32 ? 2 * q + dy : 2 * q;
-                const int mshift = SPP == 32 ? smp : dy * SPP + smp;
-                auto covers = [&](int p) -> bool { return (cmw[2 * CPR * p + moff] >> mshift) & 1u; };
+                // register p's lanes are its columns' sample bits in column order (lane
+                // q * 2N + dy * N + s is bit dy * N + s of column p * CPR + q; N = 32: the
+                // rows' words cb, cbh), so the covered lanes form one wave-wide mask read
+                // from the columns' lanes -- the select takes it as a lane mask directly
+                const uint32_t cbg = cb[g], cbhg = cbh[g];
+                auto covers = [&](int p) -> bool {
+                    uint64_t m;
+                    if (SPP == 32) {
+                        m = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)cbg, p) |
+                            ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)cbhg, p) << 32);
+                    } else {
+                        m = 0ull;
+#pragma unroll
+                        for (int k = 0; k < CPR; ++k)
+                            m |= (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)cbg, p * CPR + k) << (2 * SPP * k);
+                    }
+                    return __builtin_amdgcn_inverse_ballot_w64(m);
+                };
                 if (!gpaint) {
-                    S.apply4(regs, [&](uint32_t st, int p) -> uint32_t {
+                    auto blend = [&](uint32_t st, int p) -> uint32_t {
                         const uint32_t nst = table_code(tbl[w][g], st);
                         return covers(p) ? nst : st;
-                    });
+                    };
+                    if (GRAD) S.apply4(regs, blend);
+                    else S.template apply<kMsBlock>(regs, blend);
                 } else {
                     S.apply1(regs, [&](uint32_t st, int p) -> uint32_t {
                         const int c = p * CPR + q;
