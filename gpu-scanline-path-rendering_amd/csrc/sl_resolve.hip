@@ -1341,8 +1341,8 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 6) k_comp
 // columns per register.  Each register that holds a pixel covered by the bucket costs one
 // wave-wide blend (its lanes select by their sample bits), so a 2x2 fragment or a span
 // column costs the same whatever its sample count, and no lane loops over samples.  Solid
-// paints run over the state registers in static blocks of four (apply_blocks): a
-// register's blend is four byte lookups in the path's table and one select whose lane mask
+// paints run over the state registers in static blocks of four registers at 32 samples,
+// two at 16 and 8 (apply_blocks): a register's blend is four byte lookups in the path's table and one select whose lane mask
 // is read straight from the column lanes' sample bits, with no register indexing.
 // ---------------------------------------------------------------------------------
 typedef uint32_t u32x32 __attribute__((ext_vector_type(32)));
@@ -1353,7 +1353,6 @@ typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
 // stays in plain VGPRs (no indexing, no per-register loop), and a block's LDS round trips
 // overlap, at the price of the block's uncovered registers (f leaves a register it does not
 // cover unchanged).
-constexpr int kMsBlock = 4;
 template <int N, int BLK, class V, class F>
 __device__ __forceinline__ void apply_blocks(V &vec, uint32_t m, int pbase, const F &f) {
 #pragma unroll
@@ -1558,16 +1557,18 @@ __device__ __forceinline__ uint32_t column_bits(const uint32_t *m, int dx, uint3
 }
 
 // Measured on the headline scene's multisample composite at 32 / 16 / 8 samples: the
-// static blocks take 10.0 / 5.7 / 4.6 ms where the indexed four-register loop took
+// static blocks take 10.0 / 5.6 / 4.15 ms where the indexed four-register loop took
 // 13.9 / 7.4 / 4.6 ms (per register it spent 14 VALU, 9 scalar and 5 LDS instructions
-// against 10 VALU and 4 LDS, the blocks' idle registers included).  Blocks of two or eight
-// registers, one- or four-bucket groups, and three waves per SIMD at 32 samples were all
-// slower (10.2 / 10.3 / 10.6 / 10.5 / 11.7 ms).  Two buckets a group with registers capped
-// for 4 / 5 / 6 waves per SIMD at 32 / 16 / 8 samples (16 bytes a lane spilled at 32).  The
-// gradient variants, whose paint arithmetic would spill under those caps, keep four-bucket
-// groups, uncapped registers and the indexed loops.
+// against 10 VALU and 4 LDS, the blocks' idle registers included).  At 32 samples, blocks
+// of two or eight registers, one- or four-bucket groups, and three or five waves per SIMD
+// were all slower (10.2 / 10.3 / 10.6 / 10.5 / 11.7 / 15.1 ms: five waves spill 176 bytes
+// a lane); blocks of four cost 5.7 / 4.3 ms at 16 / 8 samples.  Two buckets a group with
+// registers capped for 4 / 5 / 7 waves per SIMD at 32 / 16 / 8 samples (16 bytes a lane
+// spilled at 32; six waves at 16 samples: 6.0 ms, eight at 8: 4.6 ms).  The gradient
+// variants, whose paint arithmetic would spill under those caps, keep four-bucket groups,
+// uncapped registers and the indexed loops.
 template <int SPP, bool SRGB, bool GRAD>
-__global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 5 : 6))) k_composite_ms(CovArgs A) {
+__global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 5 : 7))) k_composite_ms(CovArgs A) {
     constexpr int NW = Words<SPP>::n;
     constexpr int CPR = 64 / (2 * SPP);
     constexpr int NREG = 64 / CPR;
@@ -1707,7 +1708,7 @@ __global__ void __launch_bounds__(256, GRAD ? 1 : (SPP == 32 ? 4 : (SPP == 16 ? 
                         return covers(p) ? nst : st;
                     };
                     if (GRAD) S.apply4(regs, blend);
-                    else S.template apply<kMsBlock>(regs, blend);
+                    else S.template apply<SPP == 32 ? 4 : 2>(regs, blend);
                 } else {
                     S.apply1(regs, [&](uint32_t st, int p) -> uint32_t {
                         const int c = p * CPR + q;
