@@ -169,19 +169,10 @@ __global__ void k_transform(uint32_t V, const float2 *__restrict__ vtx, const ui
     float oy = (o[1] - h2) * z / (o[2] + z) + h2;
     int xf = ox < 0 ? 0 : (ox <= w ? 1 : 2);
     int yf = oy < 0 ? 0 : (oy <= h ? 1 : 2);
-    // region bit per the 3x3 layout of ras_define.h:104-117; inside sets bits 0 and 7
-    uint32_t bits;
-    switch ((yf << 4) | xf) {
-    case 0x00: bits = 1u << 0; break;
-    case 0x01: bits = 1u << 1; break;
-    case 0x02: bits = 1u << 2; break;
-    case 0x10: bits = 1u << 3; break;
-    case 0x11: bits = (1u << 0) | (1u << 7); break;
-    case 0x12: bits = 1u << 4; break;
-    case 0x20: bits = 1u << 5; break;
-    case 0x21: bits = 1u << 6; break;
-    default: bits = 1u << 7; break;
-    }
+    // region bit per the 3x3 layout of ras_define.h:104-117: regions r = 3 yf + xf take bits
+    // 0, 1, 2, 3, -, 4, 5, 6, 7 in order; the inside (r = 4) sets bits 0 and 7
+    const int r = 3 * yf + xf;
+    const uint32_t bits = r == 4 ? ((1u << 0) | (1u << 7)) : 1u << (r - (r > 4 ? 1 : 0));
     // OR the region bits into the path's flags: one atomic per distinct path of the wave
     const int lane = threadIdx.x & 63;
     uint64_t pend = __ballot(live);
@@ -190,9 +181,11 @@ __global__ void k_transform(uint32_t V, const float2 *__restrict__ vtx, const ui
         const uint32_t lp = (uint32_t)__builtin_amdgcn_readlane((int)p, leader);
         const bool mine = ((pend >> lane) & 1ull) && p == lp;
         const uint64_t grp = __ballot(mine);
-        uint32_t acc = mine ? bits : 0u;
+        // the group's OR of the eight region bits, one ballot a bit (scalar results, no
+        // cross-lane data movement)
+        uint32_t acc = 0u;
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) acc |= (uint32_t)__shfl_xor((int)acc, d, 64);
+        for (int b = 0; b < 8; ++b) acc |= __ballot(mine && ((bits >> b) & 1u)) ? (1u << b) : 0u;
         if (lane == leader) atomicOr(&pflags[lp], acc);
         pend &= ~grp;
     }
