@@ -43,6 +43,42 @@ __device__ __forceinline__ uint64_t lsr64(uint64_t v, int32_t s) { return (s < 0
 template <class T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 template <class T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 
+// Wave-wide inclusive scans with DPP row shifts and row broadcasts (no LDS round trips);
+// lane 63 ends with the whole wave's reduction.  Every lane of the wave must be active.
+struct OpAdd { __device__ int operator()(int a, int b) const { return a + b; } };
+struct OpMin { __device__ int operator()(int a, int b) const { return a < b ? a : b; } };
+struct OpMax { __device__ int operator()(int a, int b) const { return a > b ? a : b; } };
+struct OpOr { __device__ int operator()(int a, int b) const { return a | b; } };
+struct OpMinU { __device__ int operator()(int a, int b) const { return (uint32_t)a < (uint32_t)b ? a : b; } };
+struct OpMaxU { __device__ int operator()(int a, int b) const { return (uint32_t)a > (uint32_t)b ? a : b; } };
+template <class Op>
+__device__ __forceinline__ int dpp_scan(int x, int ident, Op op) {
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+// The same scan inside segments of S lanes: S = 16 keeps to the DPP rows, S = 32 adds the
+// row_bcast:15 step (rows 1 and 3 take the last lane of rows 0 and 2), S = 64 is dpp_scan.
+template <int S, class Op>
+__device__ __forceinline__ int dpp_scan_seg(int x, int ident, Op op) {
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    if (S >= 32) x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    if (S >= 64) x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+// The reduction of x over the whole wave, in every lane.
+template <class Op>
+__device__ __forceinline__ int wave_reduce(int x, int ident, Op op) {
+    return __builtin_amdgcn_readlane(dpp_scan(x, ident, op), 63);
+}
+
 __device__ __forceinline__ float safe_rcp(float a) { return (a != 0.f) ? (1.f / a) : 0.f; }
 __device__ __forceinline__ float lerp1(float a, float b, float t) { return a + t * (b - a); }
 

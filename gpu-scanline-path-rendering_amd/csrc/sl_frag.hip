@@ -349,16 +349,8 @@ constexpr uint32_t kCountWindow = 2048;
 
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 __device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = umin(v, (uint32_t)__shfl_xor((int)v, o, 64));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = umax(v, (uint32_t)__shfl_xor((int)v, o, 64));
-    return v;
-}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) { return (uint32_t)wave_reduce((int)v, -1, OpMinU()); }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return (uint32_t)wave_reduce((int)v, 0, OpMaxU()); }
 
 template <int SPP>
 __global__ void __launch_bounds__(256) k_frag_count(SceneDev sc, const uint32_t *__restrict__ tab,

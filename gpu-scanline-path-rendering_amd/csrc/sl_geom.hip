@@ -306,16 +306,8 @@ __device__ __forceinline__ float pick5(int i, float a, float b, float c, float d
 // k_make_intersection_0 (ras_cut.h:1254-1482): monotone cuts, then per monotone piece the
 // numbers of even x / y cut lines it crosses (d_udpate_cut_range) and the curve's fragment
 // count.  Also the number of solve chains (two per piece) of the intersection pass.
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = tmin(v, __shfl_xor(v, d, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = tmax(v, __shfl_xor(v, d, 64));
-    return v;
-}
+__device__ __forceinline__ int wave_min_i(int v) { return wave_reduce(v, INT_MAX, OpMin()); }
+__device__ __forceinline__ int wave_max_i(int v) { return wave_reduce(v, INT_MIN, OpMax()); }
 
 // Range of cell rows a visible curve's fragments can fall in: a fragment's row comes from
 // its chord midpoint, which lies inside the control-point hull (rationals with w > 0), so
@@ -808,9 +800,7 @@ __device__ __forceinline__ void walk_pieces_staged(PieceSetup &p, bool live, int
         else if (ny == 0) { xs = p.t0m; ys = two1; rx = nx; ry = 0; }
         else { xs = p.sx.next(); rx = nx - 1; ys = p.t0m; ry = ny; }
     }
-    uint32_t nmax = n_loop;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor((int)nmax, d, 64));
+    const uint32_t nmax = (uint32_t)wave_reduce((int)n_loop, 0, OpMaxU());
     uint32_t last = fb(-1.f);
     for (uint32_t it0 = 0; it0 < nmax; it0 += kStageSteps) {
 #pragma unroll

@@ -24,31 +24,6 @@ namespace {
 
 template <int SPP> struct Words { static constexpr int n = SPP == 8 ? 1 : (SPP == 16 ? 2 : 4); };
 
-__device__ __forceinline__ int wave_incl_scan_i(int v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = tmin(v, __shfl_xor(v, d, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = tmax(v, __shfl_xor(v, d, 64));
-    return v;
-}
-
 // fill-rule test of k_mark_merged_fragment_and_span (.cu:511)
 __device__ __forceinline__ bool fill_test(int rule, int8_t w) {
     return (rule == 0 && w != 0) || (rule == 1 && (w & 1));
@@ -218,34 +193,6 @@ struct ResolveArgs {
     uint32_t *big_count;
     uint32_t *err;
 };
-
-// Wave-wide inclusive scans / reductions with DPP row shifts and row broadcasts.
-template <class Op>
-__device__ __forceinline__ int dpp_scan(int x, int ident, Op op) {
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x111, 0xf, 0xf, false));  // row_shr:1
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x112, 0xf, 0xf, false));  // row_shr:2
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x114, 0xf, 0xf, false));  // row_shr:4
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x118, 0xf, 0xf, false));  // row_shr:8
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
-    return x;
-}
-// The same scan inside segments of S lanes: S = 16 keeps to the DPP rows, S = 32 adds the
-// row_bcast:15 step (rows 1 and 3 take the last lane of rows 0 and 2), S = 64 is dpp_scan.
-template <int S, class Op>
-__device__ __forceinline__ int dpp_scan_seg(int x, int ident, Op op) {
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x111, 0xf, 0xf, false));  // row_shr:1
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x112, 0xf, 0xf, false));  // row_shr:2
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x114, 0xf, 0xf, false));  // row_shr:4
-    x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x118, 0xf, 0xf, false));  // row_shr:8
-    if (S >= 32) x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
-    if (S >= 64) x = op(x, __builtin_amdgcn_update_dpp(ident, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
-    return x;
-}
-struct OpAdd { __device__ int operator()(int a, int b) const { return a + b; } };
-struct OpMin { __device__ int operator()(int a, int b) const { return a < b ? a : b; } };
-struct OpMax { __device__ int operator()(int a, int b) const { return a > b ? a : b; } };
-struct OpOr { __device__ int operator()(int a, int b) const { return a | b; } };
 
 // 64-pixel tiles spanned by [lo, hi) as bits of a 64-bit mask (all tiles when the canvas
 // is wider than 64 tiles), OR-reduced over the wave into lane 63.
