@@ -90,6 +90,16 @@ def test_multisample_output(rasterizer, spp):
     compare(rasterizer, scene, 256, 256, spp, ms=True, label="ms")
 
 
+@pytest.mark.parametrize("spp", [8, 16, 32])
+@pytest.mark.parametrize("srgb", [True, False])
+def test_multisample_mixed(rasterizer, spp, srgb):
+    """Per-sample output of translucent mixed paths (both fill rules, partly covered
+    samples at every edge) in both colour encodings: each sample count's register layout
+    (1, 2 or 4 columns a state register) and block size of the compositor."""
+    scene = scanline.generate(scanline.scene.KIND_MIXED, 320, 200, 200, 0x5CA1AB1E ^ 11)
+    compare(rasterizer, scene, 320, 200, spp, srgb=srgb, ms=True, label="ms-mixed")
+
+
 def test_odd_canvas_and_transform(rasterizer):
     """Odd width/height (clipped last column/row) and a non-identity projective transform."""
     scene = scanline.generate(scanline.scene.KIND_MIXED, 301, 257, 150, 99)
