@@ -90,6 +90,38 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t *in, 
     }
 }
 
+// The down-sweep for scans of at most kScanDirectTiles tiles: each block sums the tile
+// totals before its own (a few hundred L2-resident words) instead of a separate pass
+// scanning them, and the last block writes the grand total to out[n].
+constexpr size_t kScanDirectTiles = 1024;
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_down_direct(const uint32_t *in, uint32_t *out, size_t n,
+                                                                    const uint32_t *tile_sum) {
+    __shared__ uint32_t lds[4];
+    uint32_t pre = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) pre += tile_sum[i];
+    uint32_t tile_off;
+    block_excl_scan(pre, tile_off, lds);
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        size_t i = base + k;
+        v[k] = i < n ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(s, tot, lds) + tile_off;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        size_t i = base + k;
+        if (i < n) out[i] = ex;
+        ex += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tile_off + tot;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -955,6 +987,11 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *t
         return;
     }
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, n, tmp);
+    if (ntiles <= kScanDirectTiles) {
+        hipLaunchKernelGGL(k_scan_down_direct, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, out, n,
+                           (const uint32_t *)tmp);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tmp, ntiles, out + n);
     hipLaunchKernelGGL(k_scan_down, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in, out, n,
                        (const uint32_t *)tmp);
