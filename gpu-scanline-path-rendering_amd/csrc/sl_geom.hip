@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t *in, 
 // The down-sweep for scans of at most kScanDirectTiles tiles: each block sums the tile
 // totals before its own (a few hundred L2-resident words) instead of a separate pass
 // scanning them, and the last block writes the grand total to out[n].
-constexpr size_t kScanDirectTiles = 1024;
+constexpr size_t kScanDirectTiles = 4096;
 
 __global__ void __launch_bounds__(kScanThreads) k_scan_down_direct(const uint32_t *in, uint32_t *out, size_t n,
                                                                     const uint32_t *tile_sum) {
