@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t *in, 
 }
 
 // The down-sweep for scans of at most kScanDirectTiles tiles: each block sums the tile
-// totals before its own (a few hundred L2-resident words) instead of a separate pass
+// totals before its own (at most 4095 L2-resident words) instead of a separate pass
 // scanning them, and the last block writes the grand total to out[n].
 constexpr size_t kScanDirectTiles = 4096;
 
