@@ -1289,8 +1289,9 @@ __global__ void __launch_bounds__(256, (kCovGroup == 4 || !SRGB) ? 8 : 6) k_comp
 // wave-wide blend (its lanes select by their sample bits), so a 2x2 fragment or a span
 // column costs the same whatever its sample count, and no lane loops over samples.  Solid
 // paints run over the state registers in static blocks of four registers at 32 samples,
-// two at 16 and 8 (apply_blocks): a register's blend is four byte lookups in the path's table and one select whose lane mask
-// is read straight from the column lanes' sample bits, with no register indexing.
+// two at 16 and 8 (apply_blocks): a register's blend is four byte lookups in the path's
+// table and one select whose lane mask is read straight from the column lanes' sample
+// bits, with no register indexing.
 // ---------------------------------------------------------------------------------
 typedef uint32_t u32x32 __attribute__((ext_vector_type(32)));
 typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
