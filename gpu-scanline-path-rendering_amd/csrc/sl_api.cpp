@@ -465,7 +465,7 @@ int render(sl_handle *h, hipStream_t s, bool replay, Arena &A, uint8_t *out_imag
         if (h->timing && !replay) (void)hipEventRecord(h->ev[i], s);
     };
     mark(0);
-    sl::launch_fill_u32(f.err_flag, 0u, 1, s);
+    sl::launch_fill_u32(f.big_count, 0u, 2, s);  // the big-bucket count and the error flag
     sl::launch_path_init(sc, f, s);
     sl::launch_transform(sc, f, s);
     if ((rc = check_launch(h, "transform"))) return rc;

@@ -126,6 +126,14 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+// two arrays of the same length in one launch
+__global__ void k_fill2_u32(uint32_t *p, uint32_t *q, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i] = v;
+        q[i] = v;
+    }
+}
 __global__ void k_fill_u8(uint8_t *p, uint8_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1002,6 +1010,10 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *t
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_fill2_u32(uint32_t *p, uint32_t *q, uint32_t v, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill2_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, q, v, n);
 }
 void launch_valu_probe(int blocks, int iters, float *out, hipStream_t s) {
     hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, s, iters, 0.5f, out);

@@ -211,6 +211,7 @@ struct ChordScene {
 float chord_ease(int time_ms);
 void launch_chord_frame(const ChordInterval *d, int n_diagrams, uint32_t P, const ChordScene &out, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+void launch_fill2_u32(uint32_t *p, uint32_t *q, uint32_t v, size_t n, hipStream_t s);
 
 // static scratch of the allocation-free frame (sl_noalloc.hip): its size, and its device
 // address on `device` (which must be current; looked up once per device)

@@ -1887,8 +1887,7 @@ void launch_resolve(const SceneDev &sc, const Tables &t, Frame &f, hipStream_t s
     A.items = f.items; A.imask = f.keep_masks ? f.imask : nullptr; A.icov = f.icov; A.bicnt = f.bicnt;
     A.bmcnt = f.bmcnt; A.bext = f.bext;
     A.btile = f.btile;
-    A.big_list = f.big_list; A.big_count = f.big_count; A.err = f.err_flag;
-    launch_fill_u32(f.big_count, 0u, 1, s);
+    A.big_list = f.big_list; A.big_count = f.big_count; A.err = f.err_flag;  // cleared at the frame's start
     const unsigned g1 = (unsigned)std::min<int64_t>(f.NB, f.resolve_grid > 0 ? f.resolve_grid : kResolveBlocks);
     const unsigned gc = f.class_grid > 0 ? (unsigned)f.class_grid : 4096u;
     dim3 b64(64);
@@ -1913,8 +1912,7 @@ void launch_scene_carry(const SceneDev &sc, Frame &f, hipStream_t s) {
 
 void launch_order_rows(const SceneDev &sc, Frame &f, hipStream_t s) {
     const int64_t R = f.R;
-    launch_fill_u32(f.rowbeg, 0u, (size_t)R, s);
-    launch_fill_u32(f.rowend, 0u, (size_t)R, s);
+    launch_fill2_u32(f.rowbeg, f.rowend, 0u, (size_t)R, s);
     if (!f.NB) return;
     hipLaunchKernelGGL(k_nz_flags, dim3(blocks_for(f.NB, 256)), dim3(256), 0, s, f.NB, (const uint32_t *)f.bicnt, f.nz_flag);
     scan_exclusive_u32(f.nz_flag, f.nz_off, (size_t)f.NB, f.scan_tmp, s);
