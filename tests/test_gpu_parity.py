@@ -110,6 +110,18 @@ def test_odd_canvas_and_transform(rasterizer):
     compare(rasterizer, scene, 301, 257, 32, transform=m, label="transform")
 
 
+@pytest.mark.parametrize("spp", [8, 16, 32])
+def test_odd_canvas_multisample(rasterizer, spp):
+    """Per-sample output on a canvas whose last 64-pixel tile and last cell row are partial
+    (301 x 257), under the same projective transform."""
+    scene = scanline.generate(scanline.scene.KIND_MIXED, 301, 257, 150, 99)
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0], m[1, 1] = 0.9, 1.1
+    m[3, 0], m[3, 1] = 7.25, -3.5
+    m[1, 0] = 0.05
+    compare(rasterizer, scene, 301, 257, spp, ms=True, transform=m, label="odd-ms")
+
+
 def test_hand_shapes(rasterizer):
     """Rectangles on odd / integral coordinates, nested squares under both rules, a star,
     a rational arc and curves leaving the canvas."""
